@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Headline benchmark: converged BFGS starts per second on the 150-D Gaussian.
+
+Workload (one step = one zeus::Zeus call, PSO -> BFGS -> argmin -> results to
+the host): f(x) = 0.5 x^T C x with C_ii = 1 + 149 * 0.5, C_ij = 0.5, box
+[-5, 5]^150, 4096 starts, 10 PSO iterations, MAX_ITER 10000, tol 1e-8,
+seed 42, 100 required convergences (early stop), FP64, adjoint gradient.
+C is uploaded once before the timed region (inputs resident in HBM).
+
+Also measured on the same GPU: the dual-number gradient on the same call
+(the t_dual / t_adjoint ratio), the no-early-stop throughput (required =
+N + 1), the BFGS kernel against the FP64 peak, and the CPU oracle on a
+bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "converged BFGS starts/sec @tol 1e-8, 150-D Gaussian: adjoint vs dual-AD on 1 GPU"
+D, N, PSO_ITER, MAX_ITER, REQUIRED, TOL, SEED = 150, 4096, 10, 10000, 100, 1e-8, 42
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gaussian150_bfgs_group_traffic.txt")
+
+
+def gauss_matrix():
+    C = np.full((D, D), 0.5)
+    np.fill_diagonal(C, 1.0 + (D - 1) * 0.5)
+    return C
+
+
+def bfgs_flops(results):
+    """Algorithmic FP64 flops of the BFGS kernel over the given calls:
+    per start-iteration (2E + 8) D^2 with E line-search evaluations (each a
+    D^2 product with the fused value/gradient), one D^2 pass over H for
+    v = H g, and the rank-2 update; plus 2 D^2 for each start's initial
+    value/gradient."""
+    f = 0.0
+    for r in results:
+        f += (2.0 * r.sum_ls_evals + 8.0 * r.sum_iter) * D * D + 2.0 * D * D * N
+    return f
+
+
+def read_traffic():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def cpu_baseline(threads: int):
+    """Oracle (reference-literal O(D^4) update, dual gradient) on a bounded
+    sample: PSO over all 4096 particles, then BFGS from the first K = 2T
+    pBest starts to convergence (no early stop).  converged/s is
+    K_converged / (t_BFGS + t_PSO * K / 4096)."""
+    from oracle import binding as orc
+    C = gauss_matrix().reshape(-1)
+    t0 = time.perf_counter()
+    pbx, _, _, _ = orc.pso(orc.GAUSSIAN, D, -5.0, 5.0, N, PSO_ITER, SEED, params=C)
+    t_pso = time.perf_counter() - t0
+    K = 2 * threads
+    t0 = time.perf_counter()
+    r = orc.run(orc.GAUSSIAN, D, -5.0, 5.0, K, MAX_ITER, PSO_ITER, K + 1, TOL, SEED, params=C,
+                x0=pbx[:K], grad=orc.GRAD_DUAL, update_mode=orc.UPDATE_LITERAL,
+                threads=threads, early_stop=False)
+    t_bfgs = time.perf_counter() - t0
+    conv = int((r.status == 1).sum())
+    t_eff = t_bfgs + t_pso * K / N
+    return {"value": conv / t_eff, "unit": "converged starts/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"oracle reference-literal (O(D^4) update, dual gradient), OpenMP "
+                       f"{threads} threads: PSO over {N} particles ({t_pso:.2f} s) then BFGS "
+                       f"from the first {K} pBest starts ({t_bfgs:.2f} s, {conv} converged)")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--dual-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU only (multi-GPU sharding is a design "
+              "note, not a measured path); run with --gpus 1", file=sys.stderr)
+        return 2
+    if a.steps < 1 or a.warmup < 0:
+        print("bench.py: --steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        return 2
+
+    import zeus_amd as z
+    from zeus_amd import _capi as capi
+
+    f = z.Gaussian(gauss_matrix())
+    f.device_params()  # upload C once
+
+    def call(grad, required):
+        return z.Zeus(f, -5.0, 5.0, N, MAX_ITER, PSO_ITER, required, "gaussian", TOL, SEED, 0,
+                      gradient=grad)
+
+    def timed(grad, required, steps, warmup):
+        for _ in range(warmup):
+            call(grad, required)
+        capi.check(capi.lib().zeus_synchronize(), "sync")
+        t0 = time.perf_counter()
+        res = [call(grad, required) for _ in range(steps)]
+        capi.check(capi.lib().zeus_synchronize(), "sync")
+        return res, time.perf_counter() - t0
+
+    adj, t_adj = timed("adjoint", REQUIRED, a.steps, a.warmup)
+    for r in adj:
+        if r.status != 1 or r.c.claimed < REQUIRED:
+            print(f"bench.py: bad result {r}", file=sys.stderr)
+            return 1
+    dual, t_dual = timed("dual", REQUIRED, a.dual_steps, 1)
+    thr, t_thr = timed("adjoint", N + 1, max(2, a.steps // 4), 1)
+
+    conv_adj = sum(r.c.claimed for r in adj)
+    value = conv_adj / t_adj
+    ms_adj = 1e3 * t_adj / len(adj)
+    ms_dual = 1e3 * t_dual / len(dual)
+    value_dual = sum(r.c.claimed for r in dual) / t_dual
+
+    # roofline of the BFGS kernel (device time from events on its stream)
+    ms_opt = float(np.mean([r.ms_opt for r in adj]))
+    flops = bfgs_flops(adj) / len(adj)
+    achieved = flops / (ms_opt * 1e-3) / 1e12
+    thr_ms_opt = float(np.mean([r.ms_opt for r in thr]))
+    thr_flops = bfgs_flops(thr) / len(thr)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "converged starts/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_adj,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "gaussian-150 multistart: 4096 starts, PSO 10, MAX_ITER 10000, "
+                               "tol 1e-8, required 100, seed 42, adjoint gradient",
+                   "dim": D, "starts": N, "required": REQUIRED, "pso_iter": PSO_ITER,
+                   "parallelism": "single GPU, one workgroup per start"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": read_traffic(),
+                     "kernel": "bfgs_group_kernel<Gaussian<150>, 1024 threads, adjoint>",
+                     "flops_per_launch": flops, "ms_per_launch": ms_opt},
+        "dual": {"value": value_dual, "ms_per_step": ms_dual,
+                 "t_dual_over_t_adjoint": ms_dual / ms_adj,
+                 "converged_per_call": dual[0].c.claimed},
+        "no_early_stop": {"value": sum(r.c.claimed for r in thr) / t_thr,
+                          "ms_per_step": 1e3 * t_thr / len(thr),
+                          "bfgs_tflops": thr_flops / (thr_ms_opt * 1e-3) / 1e12},
+        "phases_ms": {"pso": float(np.mean([r.ms_pso for r in adj])), "bfgs": ms_opt},
+        "converged_per_call": adj[0].c.claimed,
+        "best_fval": adj[0].fval,
+    }
+    if not a.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline(max(1, threads))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

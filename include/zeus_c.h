@@ -1,0 +1,207 @@
+/*
+ * zeus_c.h -- C ABI of the MI355X (gfx950) Zeus multistart optimizer
+ * (libzeus_amd.so).  Plain C: fixed-width integers, doubles and pointers; no
+ * C++ types and no exceptions cross this line.
+ *
+ * The reference (fnal-numerics/zeus) has no FFI: its boundary is the C++
+ * template zeus::Zeus (include/zeus.cuh:341-375) instantiated in the caller's
+ * translation unit.  This ABI holds everything that does not depend on a
+ * user functor: the whole PSO -> BFGS -> argmin pipeline for the built-in
+ * objectives, its PSO and argmin phases on their own, device memory, the
+ * counter-based random stream and the test-only fault hooks.  The C++
+ * front end (zeus_amd/csrc/zeus/zeus.hpp) keeps the reference's template
+ * API for user functors and calls into this ABI for the functor-free parts.
+ *
+ * Error convention: every call returns zeus_status_t: 0 ok, 3 device
+ * allocation failure, 4 launch or runtime error, negative = API misuse
+ * (see ZEUS_E_*).  Per-start status codes (0..5) are reported in
+ * zeus_result.status.  Calls are synchronous with respect to the host and
+ * one call per device may be in flight at a time.
+ */
+#ifndef ZEUS_C_H
+#define ZEUS_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t zeus_status_t;
+
+/* return codes */
+#define ZEUS_OK 0
+#define ZEUS_E_ALLOC 3
+#define ZEUS_E_LAUNCH 4
+#define ZEUS_E_ARG (-10)        /* invalid argument value                */
+#define ZEUS_E_UNSUPPORTED (-11) /* objective/dimension not instantiated */
+#define ZEUS_E_NODEVICE (-12)   /* no usable gfx950 device               */
+
+/* per-start / per-call status codes (reference README.md:27-35) */
+#define ZEUS_UNSET (-1)
+#define ZEUS_SURRENDERED 0
+#define ZEUS_CONVERGED 1
+#define ZEUS_STOPPED 2
+#define ZEUS_MALLOC 3
+#define ZEUS_KERNEL 4
+#define ZEUS_NONFINITE 5
+
+/* built-in objectives (reference examples/{rosenbrock,rastrigin,ackley,
+   gaussian}.hpp, examples/simulation/fitting/LogLikelihood.h) */
+#define ZEUS_OBJ_ROSENBROCK 1 /* params: none                              */
+#define ZEUS_OBJ_RASTRIGIN 2  /* params: none                              */
+#define ZEUS_OBJ_ACKLEY 3     /* params: none                              */
+#define ZEUS_OBJ_GAUSSIAN 4   /* params: C, dim x dim row-major            */
+#define ZEUS_OBJ_LOGLIK 5     /* params: x[n] k[n] lgamma(k+1)[n]; dim 4   */
+#define ZEUS_OBJ_MLP 6        /* params: x[m] y[m]; dim 321                */
+#define ZEUS_OBJ_SPHERE 7     /* params: none                              */
+
+#define ZEUS_GRAD_ADJOINT 0
+#define ZEUS_GRAD_DUAL 1
+
+#define ZEUS_FORM_AUTO 0   /* autotuner's choice                          */
+#define ZEUS_FORM_THREAD 1 /* one lane per start (reference sequential)   */
+#define ZEUS_FORM_GROUP 2  /* one workgroup per start (reference parallel)*/
+
+/* One optimization problem: the arguments of zeus::Zeus for a built-in
+   objective.  Replaces the template parameters/arguments of
+   include/zeus.cuh:341-358 (f, lower, upper, N, MAX_ITER, PSO_ITER,
+   requiredConverged, fun_name, tolerance, seed). */
+typedef struct zeus_problem {
+  int32_t objective;   /* ZEUS_OBJ_*                                       */
+  int32_t dim;         /* D                                                */
+  double lower, upper; /* box [lower, upper]^D                             */
+  int64_t n_starts;    /* N                                                */
+  int32_t max_iter;    /* MAX_ITER                                         */
+  int32_t pso_iter;    /* PSO_ITER (< 0: no PSO, BFGS draws its starts)    */
+  int32_t required;    /* requiredConverged (<= 0: never stop early)       */
+  uint32_t seed;       /* Philox key word 0                                */
+  uint32_t stream;     /* Philox key word 1 (distinct streams per problem) */
+  int32_t reserved;
+  double tol;          /* gradient-norm tolerance                          */
+  const double* params;/* device pointer, objective data (see ZEUS_OBJ_*)  */
+  int64_t n_params;
+  const double* x0;    /* optional device start points N x D (skips PSO)   */
+  const char* name;    /* fun_name: selects the known minimum for the
+                          "actual" count (reference src/util.cpp:11-41)    */
+} zeus_problem;
+
+typedef struct zeus_options {
+  int32_t gradient;      /* ZEUS_GRAD_*                                    */
+  int32_t form;          /* ZEUS_FORM_*                                    */
+  int32_t group_threads; /* workgroup size of the group form (0 = tuned)   */
+  int32_t nzerosteps;    /* accepted for API parity; inert (the line search
+                            never returns a zero step)                     */
+  int32_t use_graph;     /* capture the launch sequence once per batch
+                            shape and replay it as a hipGraph              */
+  int32_t flags;         /* reserved, 0                                    */
+} zeus_options;
+
+/* Per-problem outcome: the fields of zeus::Result<D> (include/util.hpp:36-54)
+   except the coordinates, which go to the best_x argument. */
+typedef struct zeus_result {
+  int32_t idx;          /* best start (argmin of fval, NaN skipped)        */
+  int32_t status;
+  int32_t iter;
+  int32_t nonfinite;    /* number of status-5 starts                       */
+  double fval;
+  double gradient_norm;
+  int32_t claimed, actual, surrendered, stopped; /* util.hpp:27-32        */
+  double ms_pso;        /* device time of the PSO phase                    */
+  double ms_opt;        /* device time of the BFGS phase                   */
+  double ms_total;      /* host wall time of the call                      */
+  int64_t sum_iter;     /* sum over starts of BFGS iterations              */
+  int64_t sum_ls_evals; /* sum over starts of line-search evaluations      */
+  int64_t sum_grads;    /* sum over starts of gradient evaluations         */
+} zeus_result;
+
+/* Optional per-start host outputs, concatenated over problems in order;
+   any pointer may be NULL.  x holds sum_i n_starts_i * dim_i doubles. */
+typedef struct zeus_start_outputs {
+  int32_t* status;
+  int32_t* iter;
+  double* fval;
+  double* gnorm;
+  double* x;
+  int32_t* ls_evals;
+  int32_t* grads;
+} zeus_start_outputs;
+
+/* The whole pipeline (PSO -> BFGS -> argmin) for a batch of problems, in one
+   stream-ordered launch sequence (captured as a hipGraph with use_graph).
+   Replaces impl::ZeusInternal (include/zeus.cuh:37-242) for built-in
+   objectives.  results[n_probs]; best_x: sum_i dim_i host doubles.
+   stream: a hipStream_t, or NULL for the library's own stream. */
+zeus_status_t zeus_run_batch(const zeus_problem* probs, int32_t n_probs,
+                             const zeus_options* opt, zeus_result* results,
+                             double* best_x, const zeus_start_outputs* starts,
+                             void* stream);
+
+/* PSO phase alone (reference pso::launch, include/pso.cuh:194-345): device
+   outputs pbest_x (N x D), pbest_val (N), gbest_x (D), gbest_val (1). */
+zeus_status_t zeus_pso_run(const zeus_problem* prob, double* pbest_x,
+                           double* pbest_val, double* gbest_x,
+                           double* gbest_val, void* stream);
+
+/* Objective values (and gradients when grad != NULL) at n points (device
+   arrays, x: n x dim), lane per point; gradient ZEUS_GRAD_*. */
+zeus_status_t zeus_eval(const zeus_problem* prob, const double* x, int64_t n,
+                        double* f, double* grad, int32_t gradient,
+                        void* stream);
+
+/* Deterministic argmin of n device doubles (NaN skipped, ties to the lowest
+   index; all-NaN -> 0).  Replaces bfgs::launchReduction's CUB ArgMin
+   (include/bfgs_common.cuh:280-319).  *idx is a host int64. */
+zeus_status_t zeus_argmin(const double* values, int64_t n, int64_t* idx,
+                          void* stream);
+
+/* Counter-based Philox4x32-10 draw in (0,1] for (seed, stream, start, draw):
+   host evaluation, and a device fill of n consecutive draws of one start
+   (replaces the cuRAND states of src/utils.cu:17-50). */
+double zeus_philox_uniform(uint32_t seed, uint32_t stream, uint64_t start,
+                           uint64_t draw);
+zeus_status_t zeus_philox_fill(double* out, int64_t n, uint32_t seed,
+                               uint32_t stream, uint64_t start,
+                               uint64_t draw0, void* stream_handle);
+
+/* Device memory (replaces CudaBuffer's cudaMalloc/cudaMemcpy,
+   include/cuda_buffer.hpp:14-209). */
+zeus_status_t zeus_alloc(void** ptr, uint64_t bytes);
+zeus_status_t zeus_free(void* ptr);
+zeus_status_t zeus_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
+zeus_status_t zeus_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+zeus_status_t zeus_memcpy_d2d(void* dst, const void* src, uint64_t bytes);
+zeus_status_t zeus_synchronize(void);
+
+/* Name of a status code ("surrendered", "converged", ...). */
+const char* zeus_status_string(int32_t status);
+
+/* Test-only fault injection: fail the allocation number `fail_alloc_at`
+   (counted from this call, 1-based; 0 disables) and/or report the next
+   `fail_launches` kernel launches as failed.  Forces status 3 / 4 paths
+   without exhausting or crashing the device. */
+zeus_status_t zeus_set_fault_hooks(int32_t fail_alloc_at, int32_t fail_launches);
+
+/* Workgroup size / form the autotuner picked for (objective, dim, gradient)
+   after the last zeus_run_batch, for inspection: *form, *threads. */
+zeus_status_t zeus_tuned_config(int32_t objective, int32_t dim,
+                                int32_t gradient, int32_t* form,
+                                int32_t* threads);
+
+/* Synthetic inputs of the benchmark configurations, generated on the host
+   from Philox streams keyed by `seed` (documented in zeus_c.hip):
+   Poisson-sampled binned dijet spectra (x, k, lgamma(k+1) for n_bins bins
+   of m in [1000, 7000] GeV, x = m/13000, one experiment), MLP training data
+   (x ~ U[-1,1], y = sin(3x) + N(0, 0.1^2)) and SPD matrices
+   C = A A^T / D + I with A ~ N(0,1). */
+zeus_status_t zeus_make_dijet(uint32_t seed, uint32_t experiment, int32_t n_bins,
+                              double* params_out /* 3 * n_bins */);
+zeus_status_t zeus_make_mlp_data(uint32_t seed, int32_t m,
+                                 double* params_out /* 2 * m */);
+zeus_status_t zeus_make_spd(uint32_t seed, uint32_t stream, int32_t dim,
+                            double* c_out /* dim * dim */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

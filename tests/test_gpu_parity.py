@@ -1,0 +1,292 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact where both sides perform the same IEEE operations in the
+same order (Philox, argmin, lane-level evaluation of objectives made of
++ - * /, the per-thread BFGS form); otherwise |df| <= 1e-8 max(1, |f|),
+||dx||_inf <= 1e-6 and equal status, with the tolerance written in each test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import zeus_amd as z
+from zeus_amd import _capi as capi
+
+pytestmark = pytest.mark.gpu
+
+F_TOL = 1e-8   # |df| <= F_TOL * max(1, |f|)
+X_TOL = 1e-6   # ||dx||_inf
+
+
+def gauss_matrix(D, off=0.5):
+    C = np.full((D, D), off)
+    np.fill_diagonal(C, 1.0 + (D - 1) * off)
+    return C
+
+
+def close_f(a, b):
+    return abs(a - b) <= F_TOL * max(1.0, abs(a), abs(b))
+
+
+# --- primitives ------------------------------------------------------------------
+def test_philox_fill_matches_host(orc):
+    got = z.philox_fill(1000, 42, 7, 12345, draw0=3)
+    want = np.array([orc.uniform(42, 7, 12345, 3 + i) for i in range(1000)])
+    assert np.array_equal(got, want)
+
+
+def test_argmin_nan_and_ties(orc):
+    rng = np.random.default_rng(2)
+    for n in (1, 7, 300, 5000):
+        v = rng.integers(0, 4, n).astype(float)
+        v[rng.random(n) < 0.3] = np.nan
+        assert z.argmin(v) == orc.argmin(v)
+    assert z.argmin([np.nan] * 5) == 0
+
+
+OBJ_CASES = [
+    ("rosenbrock2", lambda o: (z.Rosenbrock(2), o.Obj(o.ROSENBROCK, 2)), 5.0, True),
+    ("rosenbrock50", lambda o: (z.Rosenbrock(50), o.Obj(o.ROSENBROCK, 50)), 2.0, True),
+    ("sphere2", lambda o: (z.Sphere(2), o.Obj(o.SPHERE, 2)), 5.0, True),
+    ("gaussian5", lambda o: (z.Gaussian(gauss_matrix(5) + np.triu(np.ones((5, 5)), 1)),
+                             o.Obj(o.GAUSSIAN, 5, (gauss_matrix(5) + np.triu(np.ones((5, 5)),
+                                                                            1)).reshape(-1))),
+     5.0, True),
+    ("gaussian150", lambda o: (z.Gaussian(gauss_matrix(150)),
+                               o.Obj(o.GAUSSIAN, 150, gauss_matrix(150).reshape(-1))), 5.0, True),
+    ("rastrigin20", lambda o: (z.Rastrigin(20), o.Obj(o.RASTRIGIN, 20)), 5.0, False),
+    ("ackley10", lambda o: (z.Ackley(10), o.Obj(o.ACKLEY, 10)), 30.0, False),
+]
+
+
+@pytest.mark.parametrize("name,make,box,exact", OBJ_CASES, ids=[c[0] for c in OBJ_CASES])
+def test_eval_value_and_gradients(orc, name, make, box, exact):
+    zf, of = make(orc)
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-box, box, (16, zf.dim))
+    f, ga = z.evaluate(zf, X, gradient="adjoint")
+    _, gd = z.evaluate(zf, X, gradient="dual")
+    for i in range(X.shape[0]):
+        fo = of.value(X[i])
+        if exact:
+            assert f[i] == fo
+        else:
+            assert f[i] == pytest.approx(fo, rel=1e-14, abs=1e-13)
+        _, go = of.value_grad_adjoint(X[i])
+        gdo = of.grad_dual(X[i])
+        sc = max(1.0, np.abs(go).max())
+        assert np.abs(ga[i] - go).max() <= 1e-12 * sc
+        assert np.abs(gd[i] - gdo).max() <= 1e-12 * sc
+        assert np.abs(ga[i] - gd[i]).max() <= 1e-12 * sc
+
+
+def test_eval_loglik_and_mlp(orc):
+    P = z.make_dijet(42, 1)
+    zf, of = z.LogLikelihood.from_params(P), orc.Obj(orc.LOGLIK, 4, P)
+    X = np.array([[3.2, 10.0, 5.0, 0.1], [2.5, 9.0, 4.8, 0.2], [4.0, 11.0, 5.2, 0.05]])
+    f, ga = z.evaluate(zf, X, "adjoint")
+    _, gd = z.evaluate(zf, X, "dual")
+    for i in range(3):
+        fo, go = of.value_grad_adjoint(X[i])
+        big = float(np.abs(P[100:200]).sum()) * 20  # term magnitude (NLL cancels)
+        assert abs(f[i] - fo) <= 1e-13 * big
+        assert np.abs(ga[i] - go).max() <= 1e-11 * max(1.0, np.abs(go).max())
+        assert np.abs(gd[i] - go).max() <= 1e-11 * max(1.0, np.abs(go).max())
+    D = z.make_mlp_data(42, 64)
+    zf, of = z.Mlp(D[:64], D[64:]), orc.Obj(orc.MLP, 321, D)
+    X = np.random.default_rng(4).uniform(-1, 1, (3, 321))
+    f, ga = z.evaluate(zf, X, "adjoint")
+    _, gd = z.evaluate(zf, X, "dual")
+    for i in range(3):
+        fo, go = of.value_grad_adjoint(X[i])
+        assert f[i] == pytest.approx(fo, rel=1e-13)
+        assert np.abs(ga[i] - go).max() <= 1e-12 * max(1.0, np.abs(go).max())
+        assert np.abs(gd[i] - go).max() <= 1e-12 * max(1.0, np.abs(go).max())
+
+
+# --- PSO ----------------------------------------------------------------------------
+def test_pso_rosenbrock_bitwise(orc):
+    for it in (0, 1, 10):
+        got = z.pso(z.Rosenbrock(2), -5, 5, 1000, it, 42)
+        want = orc.pso(orc.ROSENBROCK, 2, -5, 5, 1000, it, 42)
+        for g, w in zip(got, want):
+            assert np.array_equal(np.asarray(g), np.asarray(w)), it
+
+
+def test_pso_gaussian150_mfma(orc):
+    C = gauss_matrix(150)
+    N = 512
+    for it in (0, 3):
+        px, pv, gx, gv = z.pso(z.Gaussian(C), -5, 5, N, it, 42)
+        ox, ov, ogx, ogv = orc.pso(orc.GAUSSIAN, 150, -5, 5, N, it, 42, params=C.reshape(-1))
+        assert np.abs(pv - ov).max() <= 1e-12 * np.abs(ov).max()
+        assert np.abs(px - ox).max() <= 1e-9
+        assert np.abs(gx - ogx).max() <= 1e-9 and gv == pytest.approx(ogv, rel=1e-12)
+
+
+# --- BFGS: reference known answers ----------------------------------------------------
+@pytest.mark.parametrize("form", ["thread", "group"])
+@pytest.mark.parametrize("grad", ["adjoint", "dual"])
+def test_reference_launch_kats(form, grad):
+    # test/bfgs_tests.cu:266-329
+    r = z.Zeus(z.Rastrigin(2), -5, 5, 1, 10, 0, 1, "rastrigin", 1e-6, 123, 0, form=form,
+               gradient=grad, x0=[0.0, 0.0])
+    assert r.status == 1 and r.iter == 0 and r.fval == 0.0
+    # test/bfgs_tests.cu:340-389
+    r = z.Zeus(z.Sphere(2), -10, 10, 1, 100, 0, 1, "quad", 1e-6, 123, 0, form=form,
+               gradient=grad, x0=[4.5, 4.5])
+    assert r.status == 1 and abs(r.fval) < 1e-6 and np.abs(r.coordinates).max() < 1e-6
+    # test/bfgs_tests.cu:391-453 (tol 1e-17 needs an exact landing on (1, 1))
+    r = z.Zeus(z.Rosenbrock(2), -5, 5, 1, 1000, 0, 1, "rosenbrock", 1e-17, 123, 0, form=form,
+               gradient=grad, x0=[4.5, 4.5])
+    assert r.status == 1 and r.iter < 1000 and abs(r.fval) < 1e-4
+    assert np.abs(r.coordinates - 1.0).max() < 1e-6
+
+
+# --- BFGS: whole runs against the oracle ----------------------------------------------
+@pytest.mark.parametrize("grad", ["adjoint", "dual"])
+def test_rosenbrock_thread_form_bitwise(orc, grad):
+    """C1 workload (no early stop): the per-thread form repeats the oracle's
+    operations exactly, so every start's record is bit-identical."""
+    N = 1024
+    r = z.Zeus(z.Rosenbrock(2), -5, 5, N, 10000, 10, N + 1, "rosenbrock", 1e-8, 42, 0,
+               form="thread", gradient=grad, per_start=True)
+    o = orc.run(orc.ROSENBROCK, 2, -5, 5, N, 10000, 10, N + 1, 1e-8, 42,
+                grad=orc.GRAD_ADJOINT if grad == "adjoint" else orc.GRAD_DUAL)
+    ps = r.per_start
+    assert np.array_equal(ps["status"], o.status)
+    assert np.array_equal(ps["iter"], o.iter)
+    assert np.array_equal(ps["fval"], o.fval)
+    assert np.array_equal(ps["x"], o.x)
+    assert np.array_equal(ps["gnorm"], o.gnorm)
+    assert r.idx == o.best["idx"] and r.status == o.best["status"]
+    assert r.c.claimed == o.best["claimed"]
+
+
+def test_rosenbrock_group_form_tolerance(orc):
+    N = 512
+    r = z.Zeus(z.Rosenbrock(2), -5, 5, N, 10000, 10, N + 1, "rosenbrock", 1e-8, 42, 0,
+               form="group", per_start=True)
+    o = orc.run(orc.ROSENBROCK, 2, -5, 5, N, 10000, 10, N + 1, 1e-8, 42)
+    ps = r.per_start
+    both = (ps["status"] == 1) & (o.status == 1)
+    assert (ps["status"] == o.status).mean() >= 0.97
+    assert both.sum() >= 0.9 * N
+    assert np.abs(ps["x"][both] - o.x[both]).max() <= X_TOL
+    assert close_f(r.fval, o.best["fval"]) and r.status == o.best["status"] == 1
+    assert np.abs(r.coordinates - 1.0).max() <= X_TOL
+
+
+@pytest.mark.parametrize("threads", [256, 1024])
+@pytest.mark.parametrize("grad", ["adjoint", "dual"])
+def test_gaussian150_group_form(orc, grad, threads):
+    """C2 problem at 64 starts, no early stop: every start converges to the
+    origin; per-start statuses and minima match the oracle."""
+    C = gauss_matrix(150)
+    N = 64
+    r = z.Zeus(z.Gaussian(C), -5, 5, N, 10000, 10, N + 1, "gaussian", 1e-8, 42, 0,
+               form="group", group_threads=threads, gradient=grad, per_start=True)
+    o = orc.run(orc.GAUSSIAN, 150, -5, 5, N, 10000, 10, N + 1, 1e-8, 42, params=C.reshape(-1))
+    ps = r.per_start
+    assert np.array_equal(ps["status"], o.status) and set(o.status) == {1}
+    assert np.abs(ps["iter"] - o.iter).max() <= 1
+    assert all(close_f(a, b) for a, b in zip(ps["fval"], o.fval))
+    assert np.abs(ps["x"] - o.x).max() <= X_TOL
+    assert r.status == 1 and close_f(r.fval, o.best["fval"])
+    assert np.abs(r.coordinates - o.best_x).max() <= X_TOL
+
+
+def test_gaussian150_headline_early_stop(orc):
+    """The headline call (4096 starts, 100 required): the best result, its
+    status and every converged start agree with the oracle's unstopped run."""
+    C = gauss_matrix(150)
+    N = 4096
+    r = z.Zeus(z.Gaussian(C), -5, 5, N, 10000, 10, 100, "gaussian", 1e-8, 42, 0,
+               per_start=True)
+    ps = r.per_start
+    assert r.status == 1 and r.c.claimed >= 100
+    assert set(np.unique(ps["status"])) <= {1, 2}
+    assert r.c.claimed + r.c.stopped == N
+    assert r.c.actual == r.c.claimed  # every converged start is within 0.5 of 0
+    conv = np.where(ps["status"] == 1)[0]
+    sample = conv[:: max(1, len(conv) // 48)]
+    o = orc.run(orc.GAUSSIAN, 150, -5, 5, N, 10000, 10, N + 1, 1e-8, 42, params=C.reshape(-1))
+    assert np.abs(ps["iter"][sample] - o.iter[sample]).max() <= 1
+    assert np.abs(ps["x"][sample] - o.x[sample]).max() <= X_TOL
+    assert abs(r.fval) <= F_TOL and np.abs(r.coordinates).max() <= X_TOL
+    # a stopped start stopped before the oracle's convergence iteration
+    st = ps["status"] == 2
+    assert np.all(ps["iter"][st] <= o.iter[st])
+
+
+def test_early_stop_counts(orc):
+    N = 2048
+    r = z.Zeus(z.Sphere(2), -5, 5, N, 100, 2, 5, "sphere", 1e-10, 9, 0, form="thread",
+               per_start=True)
+    ps = r.per_start
+    assert r.c.claimed >= 5 and r.c.claimed + r.c.stopped == N
+    o = orc.run(orc.SPHERE, 2, -5, 5, N, 100, 2, N + 1, 1e-10, 9)
+    conv = ps["status"] == 1
+    assert np.array_equal(ps["x"][conv], o.x[conv])  # converged starts are bit-exact
+    assert np.all(ps["iter"][~conv] <= o.iter[~conv])
+
+
+def test_status_codes_and_fault_hooks():
+    r = z.Zeus(z.Rosenbrock(2), -5, 5, 8, 2, 0, 10, "rosenbrock", 1e-12, 42, 0, form="thread",
+               per_start=True)
+    assert set(r.per_start["status"]) == {0} and set(r.per_start["iter"]) == {2}
+    P = z.make_dijet(42, 0)
+    r = z.Zeus(z.LogLikelihood.from_params(P), 0, 10, 1, 50, 0, 1, "poisson", 1e-8, 1, 0,
+               form="thread", x0=[0.0, 10.0, 5.0, 0.1])
+    assert r.status == 5
+    z.set_fault_hooks(fail_alloc_at=1)
+    r = z.Zeus(z.Rosenbrock(2), -5, 5, 64, 100, 2, 10, "rosenbrock", 1e-8, 42, 0)
+    assert r.status == 3
+    z.set_fault_hooks(fail_launches=1)
+    r = z.Zeus(z.Rosenbrock(2), -5, 5, 64, 100, 2, 10, "rosenbrock", 1e-8, 42, 0)
+    assert r.status == 4
+    z.set_fault_hooks(0, 0)
+    r = z.Zeus(z.Rosenbrock(2), -5, 5, 64, 1000, 2, 65, "rosenbrock", 1e-8, 42, 0)
+    assert r.status == 1
+
+
+def test_batch_equals_individual_calls():
+    """Problems batched in one call give the same numbers as alone."""
+    C = z.make_spd(42, 1, 32)
+    probs = [
+        z.Problem(z.Rosenbrock(2), -5, 5, 300, 1000, 5, 301, 1e-8, 42, stream=1),
+        z.Problem(z.Gaussian(C), -5, 5, 200, 1000, 5, 201, 1e-8, 42, stream=2),
+        z.Problem(z.Rosenbrock(2), -5, 5, 150, 1000, 5, 151, 1e-8, 42, stream=3),
+        z.Problem(z.Ackley(10), -32.768, 32.768, 128, 1000, 5, 129, 1e-8, 42, stream=4),
+    ]
+    together = z.run_batch(probs, per_start=True)
+    for pr, t in zip(probs, together):
+        alone = z.run_batch([pr], per_start=True)[0]
+        assert np.array_equal(t.per_start["x"], alone.per_start["x"])
+        assert np.array_equal(t.per_start["status"], alone.per_start["status"])
+        assert t.idx == alone.idx and t.fval == alone.fval
+
+
+def test_dijet_fits_thread_form(orc):
+    n_fit, N = 4, 128
+    for e in range(n_fit):
+        P = z.make_dijet(42, e)
+        r = z.Zeus(z.LogLikelihood.from_params(P), 0, 10, N, 10000, 10, N + 1, "poisson",
+                   1e-8, 42, 0, form="thread", stream=e, per_start=True)
+        o = orc.run(orc.LOGLIK, 4, 0, 10, N, 10000, 10, N + 1, 1e-8, 42, stream=e, params=P)
+        assert r.status == o.best["status"]
+        assert close_f(r.fval, o.best["fval"])
+        assert np.abs(r.coordinates - o.best_x).max() <= X_TOL
+        assert (r.per_start["status"] == o.status).mean() >= 0.95
+
+
+def test_mlp_short_run(orc):
+    D = z.make_mlp_data(42, 64)
+    N = 4
+    r = z.Zeus(z.Mlp(D[:64], D[64:]), -2, 2, N, 15, 2, N + 1, "mlp", 1e-8, 42, 0,
+               form="group", per_start=True)
+    o = orc.run(orc.MLP, 321, -2, 2, N, 15, 2, N + 1, 1e-8, 42, params=D)
+    assert np.array_equal(r.per_start["status"], o.status)
+    assert np.abs(r.per_start["fval"] - o.fval).max() <= 1e-8 * max(1.0, np.abs(o.fval).max())
+    assert np.abs(r.per_start["x"] - o.x).max() <= 1e-6

@@ -1,0 +1,104 @@
+"""Build libzeus_amd.so (HIP, gfx950) in-tree.
+
+    python -m zeus_amd.build [--force] [--jobs N]
+
+Every .hip file under zeus_amd/csrc is compiled with hipcc for gfx950 only
+into zeus_amd/_build/, then linked into zeus_amd/lib/libzeus_amd.so.
+Without --force, objects newer than every source and header are reused.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import glob
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIBDIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIBDIR, "libzeus_amd.so")
+
+ARCH = "gfx950"
+# -ffp-contract=off: the fused multiply-adds of the method are written out as
+# fma() calls so that the per-thread form matches the CPU oracle bit for bit.
+# -packed-fp32-ops: no packed FP32 instructions (none are wanted here).
+HIPFLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off",
+    "-Wno-cuda-compat", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
+]
+
+
+def hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _headers():
+    hs = glob.glob(os.path.join(CSRC, "**", "*.hpp"), recursive=True)
+    hs += glob.glob(os.path.join(ROOT, "include", "*.h"))
+    return hs
+
+
+def _compile(src: str, obj: str) -> tuple[str, int, str]:
+    cmd = [hipcc(), *HIPFLAGS, "-c", src, "-o", obj]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in p.stderr.splitlines() if "packed-fp32-ops" not in l)
+    return src, p.returncode, err
+
+
+def build(force: bool = False, jobs: int | None = None, verbose: bool = True) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    newest_hdr = max((os.path.getmtime(h) for h in _headers()), default=0.0)
+    todo, objs = [], []
+    for s in srcs:
+        o = os.path.join(BUILD, os.path.basename(s)[:-4] + ".o")
+        objs.append(o)
+        stale = (force or not os.path.exists(o) or os.path.getmtime(o) < os.path.getmtime(s)
+                 or os.path.getmtime(o) < newest_hdr)
+        if stale:
+            todo.append((s, o))
+    jobs = jobs or min(8, os.cpu_count() or 1, max(1, len(todo)))
+    failed = []
+    if todo:
+        with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+            for src, rc, err in ex.map(lambda t: _compile(*t), todo):
+                name = os.path.relpath(src, ROOT)
+                if rc != 0:
+                    failed.append(name)
+                    print(f"[zeus_amd.build] FAILED {name}\n{err}", file=sys.stderr)
+                elif verbose:
+                    print(f"[zeus_amd.build] compiled {name}", flush=True)
+    if failed:
+        raise RuntimeError(f"compilation failed: {failed}")
+    if todo or not os.path.exists(LIB):
+        tmp = LIB + ".tmp"
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError(f"link failed:\n{p.stderr}")
+        os.replace(tmp, LIB)
+        if verbose:
+            print(f"[zeus_amd.build] linked {os.path.relpath(LIB, ROOT)}", flush=True)
+    return LIB
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--force", action="store_true")
+    ap.add_argument("--jobs", type=int, default=None)
+    a = ap.parse_args()
+    build(force=a.force, jobs=a.jobs)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+// ops.hpp -- the launch table of one (objective, D) class, as a template that
+// each instantiation unit expands for its dimensions.
+#pragma once
+
+#include "runtime.hpp"
+
+namespace zeus::rt {
+
+template <class F, int D, Gradient G>
+__global__ void
+eval_kernel(dev::ProblemDesc P, const double* x, int64_t n, double* f, double* g)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const F fn = dev::BuiltinProvider{}.template get<F>(P);
+  f[i] = dev::lane_value<F, D>(fn, x + i * D);
+  if (g)
+    dev::lane_gradient<F, D, G>(fn, x + i * D, g + i * D);
+}
+
+template <class F, int D>
+struct Ops {
+  static constexpr bool kMfmaPso = F::id == obj::kGaussian && D >= 16;
+  static constexpr bool kThreadOk = F::id != obj::kMlp && D <= 64;
+
+  static hipError_t pso(const dev::PsoArgs& a, int nb, hipStream_t s)
+  {
+    if (fault_take_launch())
+      return hipErrorLaunchFailure;
+    if constexpr (kMfmaPso)
+      dev::pso_gaussian_mfma_kernel<D><<<nb, 256, 0, s>>>(a);
+    else
+      dev::pso_lane_kernel<F, D, dev::BuiltinProvider><<<nb, 256, 0, s>>>(a, {});
+    return hipGetLastError();
+  }
+
+  static hipError_t bfgs_thread(const dev::ClassArgs& a, int gradient, hipStream_t s)
+  {
+    if constexpr (!kThreadOk) {
+      return hipErrorInvalidConfiguration;
+    } else {
+      if (fault_take_launch())
+        return hipErrorLaunchFailure;
+      const int nb = int((a.total_starts + 255) / 256);
+      if (gradient == 1)
+        dev::bfgs_thread_kernel<F, D, Gradient::Dual, dev::BuiltinProvider>
+          <<<nb, 256, 0, s>>>(a, {});
+      else
+        dev::bfgs_thread_kernel<F, D, Gradient::Adjoint, dev::BuiltinProvider>
+          <<<nb, 256, 0, s>>>(a, {});
+      return hipGetLastError();
+    }
+  }
+
+  template <int NT>
+  static constexpr bool kHReg = dev::RowChunk<D, NT>::kPerThread <= 48;
+
+  template <int NT, Gradient G>
+  static auto group_kernel()
+  {
+    return &dev::bfgs_group_kernel<F, D, NT, G, kHReg<NT>, dev::BuiltinProvider>;
+  }
+
+  template <int NT>
+  static hipError_t group_nt(const dev::ClassArgs& a, int gradient, int grid, hipStream_t s)
+  {
+    if (gradient == 1)
+      group_kernel<NT, Gradient::Dual>()<<<grid, NT, 0, s>>>(a, {});
+    else
+      group_kernel<NT, Gradient::Adjoint>()<<<grid, NT, 0, s>>>(a, {});
+    return hipGetLastError();
+  }
+
+  static hipError_t bfgs_group(const dev::ClassArgs& a, int gradient, int nt, int grid,
+                               hipStream_t s)
+  {
+    if (fault_take_launch())
+      return hipErrorLaunchFailure;
+    if (nt == group_nt_at(D, 0))
+      return group_nt<group_nt_at(D, 0)>(a, gradient, grid, s);
+    if constexpr (group_nt_count(D) > 1)
+      if (nt == group_nt_at(D, 1))
+        return group_nt<group_nt_at(D, 1)>(a, gradient, grid, s);
+    return hipErrorInvalidConfiguration;
+  }
+
+  template <int NT>
+  static int occ(int gradient)
+  {
+    int n = 0;
+    hipError_t e;
+    if (gradient == 1)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &n, reinterpret_cast<const void*>(group_kernel<NT, Gradient::Dual>()), NT, 0);
+    else
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &n, reinterpret_cast<const void*>(group_kernel<NT, Gradient::Adjoint>()), NT, 0);
+    return (e == hipSuccess && n > 0) ? n : 1;
+  }
+
+  static int group_blocks_per_cu(int gradient, int nt)
+  {
+    if (nt == group_nt_at(D, 0))
+      return occ<group_nt_at(D, 0)>(gradient);
+    if constexpr (group_nt_count(D) > 1)
+      if (nt == group_nt_at(D, 1))
+        return occ<group_nt_at(D, 1)>(gradient);
+    return 1;
+  }
+
+  static int64_t group_slab_per_block(int nt)
+  {
+    if (nt == group_nt_at(D, 0))
+      return kHReg<group_nt_at(D, 0)> ? 0
+                                      : int64_t(dev::RowChunk<D, group_nt_at(D, 0)>::kPerThread) *
+                                          group_nt_at(D, 0);
+    if constexpr (group_nt_count(D) > 1)
+      if (nt == group_nt_at(D, 1))
+        return kHReg<group_nt_at(D, 1)>
+                 ? 0
+                 : int64_t(dev::RowChunk<D, group_nt_at(D, 1)>::kPerThread) * group_nt_at(D, 1);
+    return 0;
+  }
+
+  static hipError_t eval(const dev::ProblemDesc& P, const double* x, int64_t n, double* f,
+                         double* g, int gradient, hipStream_t s)
+  {
+    if (fault_take_launch())
+      return hipErrorLaunchFailure;
+    const int nb = int((n + 127) / 128);
+    if (nb == 0)
+      return hipSuccess;
+    if (gradient == 1)
+      eval_kernel<F, D, Gradient::Dual><<<nb, 128, 0, s>>>(P, x, n, f, g);
+    else
+      eval_kernel<F, D, Gradient::Adjoint><<<nb, 128, 0, s>>>(P, x, n, f, g);
+    return hipGetLastError();
+  }
+
+  static const ClassOps* table()
+  {
+    static const ClassOps t = [] {
+      ClassOps o;
+      o.objective = F::id;
+      o.dim = D;
+      o.pso_ppb = kMfmaPso ? 16 : 256;
+      o.pso = &pso;
+      o.thread_ok = kThreadOk;
+      o.bfgs_thread = &bfgs_thread;
+      o.thread_slab_per_start = D <= dev::kThreadRegMaxD ? 0 : int64_t(D) * (D + 1) / 2;
+      o.n_group = group_nt_count(D);
+      for (int k = 0; k < o.n_group; ++k)
+        o.group_nt[k] = group_nt_at(D, k);
+      o.bfgs_group = &bfgs_group;
+      o.group_blocks_per_cu = &group_blocks_per_cu;
+      o.group_slab_per_block = &group_slab_per_block;
+      o.eval = &eval;
+      return o;
+    }();
+    return &t;
+  }
+};
+
+} // namespace zeus::rt

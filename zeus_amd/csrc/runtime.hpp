@@ -1,0 +1,62 @@
+// runtime.hpp -- host-side internals of libzeus_amd.so (not installed API).
+#pragma once
+
+#include "zeus/batch.hpp"
+#include "zeus/bfgs_group.hpp"
+#include "zeus/bfgs_thread.hpp"
+#include "zeus/pso.hpp"
+#include "zeus/summary.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace zeus::rt {
+
+// Launch table of one class (objective type x compile-time dimension).
+struct ClassOps {
+  int objective = 0;
+  int dim = 0;
+  int pso_ppb = 256; // particles per PSO workgroup
+  hipError_t (*pso)(const dev::PsoArgs&, int nblocks, hipStream_t) = nullptr;
+  bool thread_ok = false;
+  hipError_t (*bfgs_thread)(const dev::ClassArgs&, int gradient, hipStream_t) = nullptr;
+  int64_t thread_slab_per_start = 0; // doubles of global H per start
+  int n_group = 0;
+  int group_nt[4] = {0, 0, 0, 0};
+  hipError_t (*bfgs_group)(const dev::ClassArgs&, int gradient, int nt, int grid,
+                           hipStream_t) = nullptr;
+  int (*group_blocks_per_cu)(int gradient, int nt) = nullptr;
+  int64_t (*group_slab_per_block)(int nt) = nullptr; // doubles, 0 if resident
+  hipError_t (*eval)(const dev::ProblemDesc&, const double* x, int64_t n, double* f,
+                     double* g, int gradient, hipStream_t) = nullptr;
+};
+
+// Registry (one function per instantiation unit).
+const ClassOps* find_class(int objective, int dim);
+const ClassOps* ops_rosenbrock(int dim);
+const ClassOps* ops_rastrigin(int dim);
+const ClassOps* ops_ackley(int dim);
+const ClassOps* ops_sphere(int dim);
+const ClassOps* ops_gaussian(int dim);
+const ClassOps* ops_loglik(int dim);
+const ClassOps* ops_mlp(int dim);
+
+// Test-only fault hooks (zeus_set_fault_hooks).
+bool fault_take_launch();
+
+// Workgroup sizes of the group form offered for a dimension.
+constexpr int
+group_nt_count(int D)
+{
+  return D <= 16 ? 1 : 2;
+}
+constexpr int
+group_nt_at(int D, int k)
+{
+  if (D <= 16)
+    return 64;
+  if (D <= 64)
+    return k == 0 ? 64 : 256;
+  return k == 0 ? 256 : 1024;
+}
+
+} // namespace zeus::rt

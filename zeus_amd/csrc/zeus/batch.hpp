@@ -1,0 +1,114 @@
+// batch.hpp -- device-side description of a batch of independent problems.
+//
+// Every launch works on a *class* of problems that share the objective type
+// and the dimension D (a compile-time constant of the kernel); problems of a
+// class differ in box, start count, tolerances, seed and parameter block.
+// Starts of a class are numbered globally and contiguously:
+// problem p owns starts [start0, start0 + n_starts).
+#pragma once
+
+#include "config.hpp"
+
+namespace zeus::dev {
+
+struct ProblemDesc {
+  int32_t objective;
+  int32_t dim;
+  double lower, upper;
+  int64_t start0;   // first global start of this problem
+  int64_t n_starts;
+  int32_t required; // requiredConverged (<= 0: never stop early)
+  int32_t max_iter; // MAX_ITER
+  int32_t pso_iter; // PSO_ITER (< 0: no PSO, starts drawn in the BFGS)
+  int32_t block0;   // first PSO workgroup of this problem
+  int32_t min_kind; // known minimum for the "actual" count: 0 origin,
+                    // 1 all ones (rosenbrock), 2 (0, -1) (goldstein_price)
+  int32_t pad0;
+  uint32_t seed;    // Philox key word 0
+  uint32_t stream;  // Philox key word 1
+  double tol;
+  const double* params; // device
+  int64_t n_params;
+  const double* x0;     // optional device start points (n_starts x dim)
+  const double* aux;    // library-prepared objective data (Gaussian: Cs)
+};
+
+// Per-problem control block; zeroed before each call.
+struct Control {
+  int32_t stop_flag;       // set once `required` starts have converged
+  int32_t converged;       // number of status-1 starts so far
+  uint32_t pso_arrive;     // PSO "last workgroup" counter
+  int32_t best_idx;        // argmin start (local index), summary kernel
+  double gbest_val;        // PSO global best value
+  int32_t claimed, stopped, surrendered, nonfinite, actual, pad1;
+  int64_t sum_iter, sum_ls_evals, sum_grads;
+  int32_t best_status, best_iter; // the best start's Result fields
+  double best_fval, best_gnorm;
+};
+
+// Per-start outputs (structure of arrays over global start index).
+struct StartOut {
+  int32_t* status;
+  int32_t* iter;
+  double* fval;
+  double* gnorm;
+  double* x;        // n x D
+  int32_t* ls_evals; // objective evaluations made by line searches
+  int32_t* grads;    // gradient evaluations
+};
+
+struct ClassArgs {
+  const ProblemDesc* probs;
+  int32_t n_probs;
+  int32_t dim;
+  int64_t total_starts;
+  Control* ctl;           // n_probs entries
+  const double* starts;   // PSO pBestX (total x D) or nullptr
+  StartOut out;
+  double* h_slab;         // group form: global H storage when not resident
+  int64_t h_slab_stride;  // doubles per workgroup slab
+};
+
+// Problem owning global start s (problems are sorted by start0).
+ZEUS_HD ZEUS_FI int
+find_problem(const ProblemDesc* probs, int n, int64_t s)
+{
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].start0 <= s)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Agent-scope atomics on the control block (vector memory instructions).
+ZEUS_D ZEUS_FI int
+load_flag(const int32_t* p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+ZEUS_D ZEUS_FI void
+store_flag(int32_t* p, int v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+ZEUS_D ZEUS_FI int
+fetch_add(int32_t* p, int v)
+{
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Count a converged start and raise the stop flag when it is the
+// `required`-th one (reference include/bfgs_common.cuh:170-208).
+ZEUS_D ZEUS_FI void
+count_converged(Control* c, int required)
+{
+  const int old = fetch_add(&c->converged, 1);
+  if (old + 1 == required)
+    store_flag(&c->stop_flag, 1);
+}
+
+} // namespace zeus::dev

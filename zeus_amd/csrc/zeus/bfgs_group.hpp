@@ -1,0 +1,523 @@
+// bfgs_group.hpp -- per-workgroup BFGS: one workgroup owns one start at a
+// time and walks a strided list of starts (persistent grid).
+//
+// Restates the reference's tile form (include/bfgs_parallel.cuh:67-285) and
+// keeps its per-start semantics, but every lane works, not only lane 0:
+//   * H lives in the row-chunk layout of block.hpp, in registers when it
+//     fits and in a per-workgroup global slab otherwise;
+//   * one read-modify-write pass over H per iteration: the rank-2 update of
+//     iteration k is applied lazily inside iteration k+1's product
+//     v = H' g_new, and since H is symmetric, u = H y = v + p and the next
+//     direction p' = -H' g follow from v, u and two dot products;
+//   * the objective is evaluated cooperatively (GroupEval below), the
+//     gradient by a hand-written adjoint or by D dual passes spread over
+//     the lanes (the reference's grad_ad_tile method, bfgs_parallel.cuh:17-54).
+// Sums are workgroup reductions in a fixed order: results are deterministic
+// but not bit-identical to the serial oracle (parity is to tolerance).
+#pragma once
+
+#include "batch.hpp"
+#include "block.hpp"
+#include "lane_ops.hpp"
+#include "philox.hpp"
+
+namespace zeus::dev {
+
+// LDS scratch handed to the evaluators.
+struct GroupScratch {
+  double* part;   // CH * D
+  double* stash;  // D
+  double* scalar; // 4
+};
+
+// ---------------------------------------------------------------------------
+// Default evaluator: one lane evaluates (works for any functor); gradient by
+// the functor's adjoint on one lane, or by dual passes spread over lanes.
+template <class F, int D, int NT, Gradient G>
+struct GroupEvalLane {
+  F f;
+  GroupScratch sc;
+  int par = 0;
+  ZEUS_D void bind(const F& ff, const ProblemDesc&) { f = ff; }
+  ZEUS_D double value(const double* x, Block<NT>&)
+  {
+    double* b = sc.scalar + par;
+    par ^= 1;
+    if (threadIdx.x == 0)
+      *b = lane_value<F, D>(f, x);
+    __syncthreads();
+    return *b;
+  }
+  ZEUS_D void gradient(const double* x, double* g, bool, Block<NT>&)
+  {
+    if constexpr (G == Gradient::Adjoint && HasAdjoint<F>) {
+      if (threadIdx.x == 0)
+        (void)f.value_and_gradient(x, g);
+    } else {
+      for (int i = threadIdx.x; i < D; i += NT)
+        g[i] = lane_dual_pass<F, D>(f, x, i);
+    }
+  }
+};
+
+// Dual gradient shared by the cooperative evaluators: pass i on lane i.
+template <class F, int D, int NT>
+ZEUS_D ZEUS_FI void
+group_dual_gradient(const F& f, const double* x, double* g)
+{
+  for (int i = threadIdx.x; i < D; i += NT)
+    g[i] = lane_dual_pass<F, D>(f, x, i);
+}
+
+// ---------------------------------------------------------------------------
+// Gaussian: Cs = (C + C^T) / 2 (prepared once per call, ProblemDesc::aux)
+// held in the row-chunk layout; f = 0.5 x.(Cs x); g = Cs x is the
+// by-product of the value, so an accepted line-search trial already carries
+// the gradient.
+template <int D, int NT, Gradient G>
+struct GroupEvalGaussian {
+  using F = obj::Gaussian<D>;
+  using Lay = RowChunk<D, NT>;
+  static constexpr bool kReg = Lay::kPerThread <= 40;
+  F f;
+  DistMatrix<D, NT, kReg> C;
+  const double* Cs = nullptr;
+  GroupScratch sc;
+
+  ZEUS_D void bind(const F& ff, const ProblemDesc& P)
+  {
+    f = ff;
+    Cs = P.aux;
+    if constexpr (kReg)
+      C.load(Cs);
+  }
+  // part <- chunk partials of Cs x
+  ZEUS_D ZEUS_FI void partials(const double* x)
+  {
+    if constexpr (kReg) {
+      C.pass(x, nullptr, nullptr, false, sc.part);
+    } else {
+      const int c = threadIdx.x % Lay::CH, r0 = threadIdx.x / Lay::CH;
+      for (int k = 0; k < Lay::R; ++k) {
+        const int row = r0 + k * Lay::RPP;
+        if (row < D) {
+          double acc = 0.0;
+          for (int l = 0; l < Lay::L; ++l) {
+            const int col = c * Lay::L + l;
+            if (col < D)
+              acc = ::fma(Cs[row * D + col], x[col], acc);
+          }
+          sc.part[c * D + row] = acc;
+        }
+      }
+    }
+  }
+  ZEUS_D double value(const double* x, Block<NT>& blk)
+  {
+    partials(x);
+    __syncthreads();
+    double t = 0.0;
+    for (int row = threadIdx.x; row < D; row += NT) {
+      const double v = chunk_total<D, NT>(sc.part, row);
+      sc.stash[row] = v;
+      t = ::fma(x[row], v, t);
+    }
+    return 0.5 * blk.sum1(t);
+  }
+  ZEUS_D void gradient(const double* x, double* g, bool at_last_value, Block<NT>& blk)
+  {
+    if constexpr (G == Gradient::Dual) {
+      group_dual_gradient<F, D, NT>(f, x, g);
+    } else {
+      if (!at_last_value) {
+        partials(x);
+        __syncthreads();
+        for (int row = threadIdx.x; row < D; row += NT)
+          sc.stash[row] = chunk_total<D, NT>(sc.part, row);
+      }
+      for (int row = threadIdx.x; row < D; row += NT)
+        g[row] = sc.stash[row];
+    }
+    (void)blk;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Separable sums over the coordinates (Rosenbrock, Rastrigin, Ackley,
+// sphere): lane i evaluates term i, the workgroup sums.
+template <class F, int D, int NT, Gradient G>
+struct GroupEvalSeparable {
+  F f;
+  GroupScratch sc;
+  ZEUS_D void bind(const F& ff, const ProblemDesc&) { f = ff; }
+
+  ZEUS_D double value(const double* x, Block<NT>& blk)
+  {
+    if constexpr (F::id == obj::kAckley) {
+      double v[2] = {0.0, 0.0};
+      for (int i = threadIdx.x; i < D; i += NT) {
+        v[0] = ::fma(x[i], x[i], v[0]);
+        v[1] += ::cos(2.0 * obj::kPi * x[i]);
+      }
+      blk.template sum<2>(v);
+      const double term1 = -20.0 * ::exp(-0.2 * ::sqrt(v[0] / int(D)));
+      const double term2 = -::exp(v[1] / int(D));
+      return term1 + term2 + 20.0 + ::exp(1.0);
+    } else {
+      double t = 0.0;
+      for (int i = threadIdx.x; i < D; i += NT)
+        t += term(x, i);
+      const double s = blk.sum1(t);
+      if constexpr (F::id == obj::kRastrigin)
+        return 10.0 * D + s;
+      else
+        return s;
+    }
+  }
+  ZEUS_D ZEUS_FI double term(const double* x, int i) const
+  {
+    if constexpr (F::id == obj::kRosenbrock) {
+      if (i >= D - 1)
+        return 0.0;
+      const double t1 = 1.0 - x[i];
+      const double t2 = x[i + 1] - x[i] * x[i];
+      return t1 * t1 + 100.0 * t2 * t2;
+    } else if constexpr (F::id == obj::kRastrigin) {
+      return x[i] * x[i] - 10.0 * ::cos(2.0 * obj::kPi * x[i]);
+    } else {
+      return x[i] * x[i];
+    }
+  }
+  ZEUS_D void gradient(const double* x, double* g, bool, Block<NT>& blk)
+  {
+    if constexpr (G == Gradient::Dual) {
+      group_dual_gradient<F, D, NT>(f, x, g);
+    } else if constexpr (F::id == obj::kRosenbrock) {
+      for (int i = threadIdx.x; i < D; i += NT) {
+        double gi = 0.0;
+        if (i < D - 1) {
+          const double t1 = 1.0 - x[i];
+          const double t2 = x[i + 1] - x[i] * x[i];
+          gi = -2.0 * t1 - (2.0 * x[i]) * (200.0 * t2);
+        }
+        if (i > 0) {
+          const double t2p = x[i] - x[i - 1] * x[i - 1];
+          gi += 200.0 * t2p;
+        }
+        g[i] = gi;
+      }
+    } else if constexpr (F::id == obj::kRastrigin) {
+      const double k = 10.0 * (2.0 * obj::kPi);
+      for (int i = threadIdx.x; i < D; i += NT)
+        g[i] = 2.0 * x[i] + k * ::sin(2.0 * obj::kPi * x[i]);
+    } else if constexpr (F::id == obj::kAckley) {
+      double v[2] = {0.0, 0.0};
+      for (int i = threadIdx.x; i < D; i += NT) {
+        v[0] = ::fma(x[i], x[i], v[0]);
+        v[1] += ::cos(2.0 * obj::kPi * x[i]);
+      }
+      blk.template sum<2>(v);
+      const double r = ::sqrt(v[0] / int(D));
+      const double e1 = ::exp(-0.2 * r);
+      const double e2 = ::exp(v[1] / int(D));
+      const double q = (4.0 * e1) / (double(int(D)) * r);
+      const double c = (2.0 * obj::kPi / int(D)) * e2;
+      for (int i = threadIdx.x; i < D; i += NT)
+        g[i] = q * x[i] + c * ::sin(2.0 * obj::kPi * x[i]);
+    } else {
+      for (int i = threadIdx.x; i < D; i += NT)
+        g[i] = 2.0 * x[i];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Poisson NLL: bins spread over the lanes.
+template <int NT, Gradient G>
+struct GroupEvalLogLik {
+  using F = obj::LogLikelihood;
+  F f;
+  GroupScratch sc;
+  ZEUS_D void bind(const F& ff, const ProblemDesc&) { f = ff; }
+  ZEUS_D double value(const double* th, Block<NT>& blk)
+  {
+    const double t0 = th[0], t1 = th[1], t2 = th[2], t3 = th[3];
+    const double tt[4] = {t0, t1, t2, t3};
+    double t = 0.0;
+    for (int i = threadIdx.x; i < f.n; i += NT)
+      t += f.bin_value(tt, i);
+    return blk.sum1(t);
+  }
+  ZEUS_D void gradient(const double* th, double* g, bool, Block<NT>& blk)
+  {
+    if constexpr (G == Gradient::Dual) {
+      group_dual_gradient<F, 4, NT>(f, th, g);
+    } else {
+      const double a = th[0], b = th[1], c = th[2], d = th[3];
+      double v[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int i = threadIdx.x; i < f.n; i += NT) {
+        const double x = f.P[i];
+        const double k = f.P[f.n + i];
+        const double lx = ::log(x);
+        const double pw = ::pow(1.0 - x, b);
+        const double qw = ::pow(x, c + d * lx);
+        const double mean = a * pw / qw;
+        const double w = 1.0 - k / mean;
+        const double wm = w * mean;
+        v[0] += w * (pw / qw);
+        v[1] += wm * ::log(1.0 - x);
+        v[2] -= wm * lx;
+        v[3] -= wm * (lx * lx);
+      }
+      blk.template sum<4>(v);
+      if (threadIdx.x < 4)
+        g[threadIdx.x] = v[threadIdx.x];
+    }
+  }
+};
+
+template <class F, int D, int NT, Gradient G>
+struct GroupEvalSelect {
+  using type = GroupEvalLane<F, D, NT, G>;
+};
+template <int D, int NT, Gradient G>
+struct GroupEvalSelect<obj::Gaussian<D>, D, NT, G> {
+  using type = GroupEvalGaussian<D, NT, G>;
+};
+template <int D, int NT, Gradient G>
+struct GroupEvalSelect<obj::Rosenbrock<D>, D, NT, G> {
+  using type = GroupEvalSeparable<obj::Rosenbrock<D>, D, NT, G>;
+};
+template <int D, int NT, Gradient G>
+struct GroupEvalSelect<obj::Rastrigin<D>, D, NT, G> {
+  using type = GroupEvalSeparable<obj::Rastrigin<D>, D, NT, G>;
+};
+template <int D, int NT, Gradient G>
+struct GroupEvalSelect<obj::Ackley<D>, D, NT, G> {
+  using type = GroupEvalSeparable<obj::Ackley<D>, D, NT, G>;
+};
+template <int D, int NT, Gradient G>
+struct GroupEvalSelect<obj::Sphere<D>, D, NT, G> {
+  using type = GroupEvalSeparable<obj::Sphere<D>, D, NT, G>;
+};
+template <int NT, Gradient G>
+struct GroupEvalSelect<obj::LogLikelihood, 4, NT, G> {
+  using type = GroupEvalLogLik<NT, G>;
+};
+
+// ---------------------------------------------------------------------------
+// LDS vectors of one workgroup.
+template <int D, int NT>
+struct GroupVectors {
+  double x[D], g[D], p[D], xt[D], gn[D], v[D];
+  double s0[D], a0[D], s1[D], a1[D]; // current / pending rank-2 pair
+  double part[RowChunk<D, NT>::CH * D];
+  double stash[D];
+  double red[8 * (NT / kWave)];
+  double scalar[4];
+  int flag;
+};
+
+template <class F, int D, int NT, Gradient G, bool HREG, class Eval>
+ZEUS_D void
+group_bfgs(GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval& ev,
+           Block<NT>& blk, const ProblemDesc& P, Control* ctl, int64_t s,
+           const double* starts, const StartOut& out)
+{
+  const int tid = threadIdx.x;
+  const int64_t sl = s - P.start0;
+  double* sc = L.s0; // s of this iteration
+  double* ac = L.a0;
+  double* sp = L.s1; // pending pair from the previous iteration
+  double* ap = L.a1;
+  bool pending = false;
+
+  H.set_identity();
+  for (int d = tid; d < D; d += NT) {
+    if (starts)
+      L.x[d] = starts[s * D + d];
+    else if (P.x0)
+      L.x[d] = P.x0[sl * D + d];
+    else
+      L.x[d] = rng::uniform_in(P.seed, P.stream, uint64_t(sl), uint64_t(d), P.lower,
+                               P.upper);
+  }
+  __syncthreads();
+  double bestVal = ev.value(L.x, blk);
+  ev.gradient(L.x, L.g, true, blk);
+  __syncthreads();
+  for (int i = tid; i < D; i += NT)
+    L.p[i] = -L.g[i];
+  int grads = 1, ls_evals = 0;
+  int status = kStatusUnset;
+  double fval = 333777.0, gnorm = 69.0;
+  int iter = 0;
+  bool done = false;
+
+  for (; iter < P.max_iter; ++iter) {
+    if (tid == 0)
+      L.flag = load_flag(&ctl->stop_flag);
+    __syncthreads();
+    if (L.flag != 0) {
+      status = kStatusStopped;
+      fval = bestVal;
+      double gg = 0.0;
+      for (int i = tid; i < D; i += NT)
+        gg = ::fma(L.g[i], L.g[i], gg);
+      gnorm = ::sqrt(blk.sum1(gg));
+      done = true;
+      break;
+    }
+    double dd = 0.0;
+    for (int i = tid; i < D; i += NT)
+      dd = ::fma(L.g[i], L.p[i], dd);
+    const double ddir = blk.sum1(dd);
+
+    double alpha = 1.0, f1 = 0.0;
+    bool accepted = false;
+    for (int t = 0; t < kLineSearchTrials; ++t) {
+      for (int j = tid; j < D; j += NT)
+        L.xt[j] = ::fma(alpha, L.p[j], L.x[j]);
+      __syncthreads();
+      f1 = ev.value(L.xt, blk);
+      ++ls_evals;
+      if (f1 <= ::fma(kArmijoC1 * alpha, ddir, bestVal) && f1 < bestVal) {
+        accepted = true;
+        break;
+      }
+      alpha *= 0.5;
+    }
+    double fnew = f1;
+    if (!accepted) {
+      for (int j = tid; j < D; j += NT)
+        L.xt[j] = ::fma(alpha, L.p[j], L.x[j]);
+      __syncthreads();
+      fnew = ev.value(L.xt, blk);
+      ++ls_evals;
+    }
+    ev.gradient(L.xt, L.gn, true, blk);
+    ++grads;
+    __syncthreads();
+
+    // v = H' g_new while applying the pending update (one pass over H)
+    H.pass(L.gn, sp, ap, pending, L.part);
+    __syncthreads();
+    double st[2] = {0.0, 0.0};
+    for (int i = tid; i < D; i += NT) {
+      const double v = chunk_total<D, NT>(L.part, i);
+      L.v[i] = v;
+      const double si = L.xt[i] - L.x[i];
+      const double yi = L.gn[i] - L.g[i];
+      sc[i] = si;
+      st[0] = ::fma(si, yi, st[0]);
+      st[1] = ::fma(yi, v + L.p[i], st[1]); // u = H y = v + p
+    }
+    blk.template sum<2>(st);
+    const double sTy = st[0];
+    const bool upd = !(::fabs(sTy) < kCurvatureSkip);
+    const bool acc = fnew < bestVal;
+    if (upd) {
+      const double rho = 1.0 / sTy;
+      const double beta = ::fma(rho * rho, st[1], rho);
+      const double hb = 0.5 * beta;
+      double q[2] = {0.0, 0.0};
+      for (int i = tid; i < D; i += NT) {
+        const double u = L.v[i] + L.p[i];
+        const double ai = ::fma(hb, sc[i], -(rho * u));
+        ac[i] = ai;
+        const double ga = acc ? L.gn[i] : L.g[i];
+        q[0] = ::fma(ai, ga, q[0]);
+        q[1] = ::fma(sc[i], ga, q[1]);
+      }
+      blk.template sum<2>(q);
+      for (int i = tid; i < D; i += NT) {
+        const double hg = acc ? L.v[i] : -L.p[i];
+        L.p[i] = -::fma(ac[i], q[1], ::fma(sc[i], q[0], hg));
+      }
+    } else {
+      for (int i = tid; i < D; i += NT)
+        L.p[i] = acc ? -L.v[i] : L.p[i];
+    }
+    if (acc) {
+      bestVal = fnew;
+      for (int i = tid; i < D; i += NT) {
+        L.x[i] = L.xt[i];
+        L.g[i] = L.gn[i];
+      }
+    }
+    double gg = 0.0;
+    for (int i = tid; i < D; i += NT)
+      gg = ::fma(L.g[i], L.g[i], gg);
+    const double nrm = ::sqrt(blk.sum1(gg));
+    // the new pair becomes pending for the next pass
+    pending = upd;
+    double* t0 = sp;
+    double* t1 = ap;
+    sp = sc;
+    ap = ac;
+    sc = t0;
+    ac = t1;
+
+    if (!::isfinite(nrm) || !::isfinite(fnew)) {
+      status = kStatusNonFinite;
+      fval = fnew;
+      gnorm = nrm;
+      done = true;
+      break;
+    }
+    if (nrm < P.tol) {
+      if (tid == 0)
+        count_converged(ctl, P.required);
+      status = kStatusConverged;
+      fval = bestVal;
+      gnorm = nrm;
+      done = true;
+      break;
+    }
+  }
+  if (!done) {
+    status = kStatusSurrendered;
+    fval = bestVal;
+    double gg = 0.0;
+    for (int i = tid; i < D; i += NT)
+      gg = ::fma(L.g[i], L.g[i], gg);
+    gnorm = ::sqrt(blk.sum1(gg));
+  }
+  if (tid == 0) {
+    out.status[s] = status;
+    out.iter[s] = iter;
+    out.fval[s] = fval;
+    out.gnorm[s] = gnorm;
+    out.ls_evals[s] = ls_evals;
+    out.grads[s] = grads;
+  }
+  for (int d = tid; d < D; d += NT)
+    out.x[s * D + d] = L.x[d];
+  __syncthreads();
+}
+
+template <class F, int D, int NT, Gradient G, bool HREG, class Prov>
+__global__ __launch_bounds__(NT) void
+bfgs_group_kernel(ClassArgs a, Prov prov)
+{
+  __shared__ GroupVectors<D, NT> L;
+  Block<NT> blk{L.red};
+  using Eval = typename GroupEvalSelect<F, D, NT, G>::type;
+  Eval ev;
+  ev.sc = GroupScratch{L.part, L.stash, L.scalar};
+  DistMatrix<D, NT, HREG> H;
+  if constexpr (!HREG)
+    H.slab = a.h_slab + int64_t(blockIdx.x) * a.h_slab_stride;
+  int bound = -1;
+  for (int64_t s = blockIdx.x; s < a.total_starts; s += gridDim.x) {
+    const int pi = find_problem(a.probs, a.n_probs, s);
+    const ProblemDesc P = a.probs[pi];
+    if (pi != bound) {
+      ev.bind(prov.template get<F>(P), P);
+      bound = pi;
+    }
+    group_bfgs<F, D, NT, G, HREG>(L, H, ev, blk, P, a.ctl + pi, s, a.starts, a.out);
+  }
+}
+
+} // namespace zeus::dev

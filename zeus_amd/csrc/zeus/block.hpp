@@ -1,0 +1,166 @@
+// block.hpp -- workgroup-level primitives for the per-workgroup BFGS.
+#pragma once
+
+#include "config.hpp"
+
+namespace zeus::dev {
+
+// Sum over the 64 lanes of a wavefront by butterfly exchange; every lane
+// ends with the same bits (each step adds the same two operands).
+ZEUS_D ZEUS_FI double
+wave_sum(double v)
+{
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Deterministic workgroup sums of K values at a time.  Two LDS slots are
+// used alternately so that back-to-back reductions need one barrier each.
+template <int NT>
+struct Block {
+  static constexpr int kWaves = NT / kWave;
+  static_assert(NT % kWave == 0, "workgroup must be whole wavefronts");
+  double* red;   // LDS, 2 * 4 * kWaves doubles
+  int parity = 0;
+
+  template <int K>
+  ZEUS_D ZEUS_FI void sum(double (&v)[K])
+  {
+    static_assert(K <= 4, "at most 4 values per reduction");
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      v[k] = wave_sum(v[k]);
+    double* r = red + parity * (4 * kWaves);
+    parity ^= 1;
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        r[k * kWaves + w] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < kWaves; ++i)
+        t += r[k * kWaves + i];
+      v[k] = t;
+    }
+  }
+  ZEUS_D ZEUS_FI double sum1(double v)
+  {
+    double a[1] = {v};
+    sum<1>(a);
+    return a[0];
+  }
+};
+
+// Row-chunk layout of a D x D matrix over NT threads: thread t owns chunk
+// c = t % CH of the rows r0 + k * RPP (r0 = t / CH, k < R); a chunk is L
+// consecutive columns.  Matrix-vector products reduce the CH chunk partial
+// sums of a row in a fixed order, so results do not depend on timing.
+template <int D, int NT>
+struct RowChunk {
+  static constexpr int CH = (NT / D) >= 1 ? (NT / D) : 1;
+  static constexpr int L = (D + CH - 1) / CH;
+  static constexpr int RPP = NT / CH;
+  static constexpr int R = (D + RPP - 1) / RPP;
+  static constexpr int kPerThread = R * L;
+};
+
+// A D x D matrix distributed over the workgroup in the row-chunk layout,
+// resident in registers (REG) or in a per-workgroup global slab whose
+// element e of thread t sits at slab[e * NT + t] (coalesced).
+template <int D, int NT, bool REG>
+struct DistMatrix {
+  using Lay = RowChunk<D, NT>;
+  double m[REG ? Lay::kPerThread : 1];
+  double* slab = nullptr;
+
+  ZEUS_D ZEUS_FI double& at(int e)
+  {
+    if constexpr (REG)
+      return m[e];
+    else
+      return slab[int64_t(e) * NT + threadIdx.x];
+  }
+
+  ZEUS_D ZEUS_FI void set_identity()
+  {
+    const int c = threadIdx.x % Lay::CH, r0 = threadIdx.x / Lay::CH;
+#pragma unroll(REG ? 64 : 1)
+    for (int k = 0; k < Lay::R; ++k) {
+      const int row = r0 + k * Lay::RPP;
+#pragma unroll(REG ? 64 : 1)
+      for (int l = 0; l < Lay::L; ++l) {
+        const int col = c * Lay::L + l;
+        at(k * Lay::L + l) = (row == col && row < D) ? 1.0 : 0.0;
+      }
+    }
+  }
+
+  // Load a row-major D x D matrix.
+  ZEUS_D ZEUS_FI void load(const double* src)
+  {
+    const int c = threadIdx.x % Lay::CH, r0 = threadIdx.x / Lay::CH;
+#pragma unroll(REG ? 64 : 1)
+    for (int k = 0; k < Lay::R; ++k) {
+      const int row = r0 + k * Lay::RPP;
+#pragma unroll(REG ? 64 : 1)
+      for (int l = 0; l < Lay::L; ++l) {
+        const int col = c * Lay::L + l;
+        at(k * Lay::L + l) = (row < D && col < D) ? src[row * D + col] : 0.0;
+      }
+    }
+  }
+
+  // part[c * D + row] = sum over the chunk of M[row][col] * x[col].
+  // With `pending`, first applies M += s a^T + a s^T (formed as
+  // s_r a_c + a_r s_c, which is symmetric bit for bit) and stores it back:
+  // one read-modify-write pass per BFGS iteration.
+  ZEUS_D ZEUS_FI void pass(const double* x, const double* s, const double* a,
+                           bool pending, double* part)
+  {
+    const int c = threadIdx.x % Lay::CH, r0 = threadIdx.x / Lay::CH;
+#pragma unroll(REG ? 64 : 1)
+    for (int k = 0; k < Lay::R; ++k) {
+      const int row = r0 + k * Lay::RPP;
+      if (row < D) {
+        const double sr = pending ? s[row] : 0.0;
+        const double ar = pending ? a[row] : 0.0;
+        double acc = 0.0;
+#pragma unroll(REG ? 64 : 1)
+        for (int l = 0; l < Lay::L; ++l) {
+          const int col = c * Lay::L + l;
+          if (col < D) {
+            double h = at(k * Lay::L + l);
+            if (pending) {
+              h = h + (sr * a[col] + ar * s[col]);
+              at(k * Lay::L + l) = h;
+            }
+            acc = ::fma(h, x[col], acc);
+          }
+        }
+        part[c * D + row] = acc;
+      }
+    }
+  }
+};
+
+// v[row] = sum_c part[c * D + row], rows owned i = tid, tid + NT, ...
+template <int D, int NT>
+ZEUS_D ZEUS_FI double
+chunk_total(const double* part, int row)
+{
+  using Lay = RowChunk<D, NT>;
+  double t = 0.0;
+#pragma unroll
+  for (int c = 0; c < Lay::CH; ++c)
+    t += part[c * D + row];
+  return t;
+}
+
+} // namespace zeus::dev

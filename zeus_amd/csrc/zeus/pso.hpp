@@ -1,0 +1,276 @@
+// pso.hpp -- particle-swarm seeding, synchronous form.
+//
+// Restates reference include/pso.cuh:17-155 (init / iteration kernels) and
+// 194-345 (launch): positions U(lo, hi], velocities U(-0.1R, 0.1R], then
+// PSO_ITER updates v' = w v + c1 r1 (pbest - x) + c2 r2 (gbest - x),
+// x' = x + v' (no clamping), pBest on strict improvement.  The result is
+// pBestX.  Designed-out hazards of the reference: gBest is frozen during an
+// iteration and replaced after it -- only on strict improvement, ties to the
+// lowest particle index -- by the last workgroup of the problem to finish
+// (wavefront-shuffle argmin, then LDS across wavefronts, then the
+// per-workgroup partials), so runs are reproducible and need no host sync.
+//
+// Buffers are start-major: X[s * D + d] for global start s of the class.
+#pragma once
+
+#include "batch.hpp"
+#include "lane_ops.hpp"
+#include "philox.hpp"
+
+namespace zeus::dev {
+
+struct PsoArgs {
+  const ProblemDesc* probs;
+  int32_t n_probs;
+  int32_t iter;        // 0 = init, t >= 1 = iteration t
+  Control* ctl;
+  double* X;
+  double* V;
+  double* pbx;
+  double* pbv;
+  double* gbx;         // n_probs x D
+  double* part_val;    // one per workgroup
+  int64_t* part_idx;
+  const int32_t* block_problem; // workgroup -> problem
+};
+
+// (value, index) order for argmin: NaN loses to any number, ties and the
+// all-NaN case go to the lowest index.
+ZEUS_HD ZEUS_FI bool
+better(double a, int64_t ia, double b, int64_t ib)
+{
+  const bool na = a != a, nb = b != b;
+  if (na != nb)
+    return nb;
+  if (!na && a != b)
+    return a < b;
+  return ia < ib;
+}
+
+ZEUS_D ZEUS_FI void
+wave_argmin(double& v, int64_t& i)
+{
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off, kWave);
+    const int64_t oi = __shfl_xor(i, off, kWave);
+    if (better(ov, oi, v, i)) {
+      v = ov;
+      i = oi;
+    }
+  }
+}
+
+// Workgroup argmin; result valid in thread 0.
+template <int NT>
+ZEUS_D ZEUS_FI void
+block_argmin(double& v, int64_t& i, double* sv, int64_t* si)
+{
+  wave_argmin(v, i);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    sv[w] = v;
+    si[w] = i;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < NT / kWave; ++k)
+      if (better(sv[k], si[k], v, i)) {
+        v = sv[k];
+        i = si[k];
+      }
+  }
+}
+
+// After the workgroup's particles are final: publish the workgroup's best,
+// and let the last workgroup of the problem fold the partials into gBest.
+template <int NT, int PPB, int D>
+ZEUS_D void
+publish_best(const PsoArgs& a, int pi, const ProblemDesc& P, double v, int64_t idx)
+{
+  __shared__ double sv[NT / kWave];
+  __shared__ int64_t si[NT / kWave];
+  __shared__ int last;
+  block_argmin<NT>(v, idx, sv, si);
+  const int nblk = int((P.n_starts + PPB - 1) / PPB); // PPB particles per workgroup
+  if (threadIdx.x == 0) {
+    a.part_val[blockIdx.x] = v;
+    a.part_idx[blockIdx.x] = idx;
+    __threadfence();
+    const uint32_t t = __hip_atomic_fetch_add(&a.ctl[pi].pso_arrive, 1u, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    last = (t == uint32_t(nblk - 1));
+  }
+  __syncthreads();
+  if (!last)
+    return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double bv = a.part_val[P.block0];
+    int64_t bi = a.part_idx[P.block0];
+    for (int b = 1; b < nblk; ++b) {
+      const double ov = a.part_val[P.block0 + b];
+      const int64_t oi = a.part_idx[P.block0 + b];
+      if (better(ov, oi, bv, bi)) {
+        bv = ov;
+        bi = oi;
+      }
+    }
+    a.ctl[pi].pso_arrive = 0u;
+    if (bv < a.ctl[pi].gbest_val) {
+      a.ctl[pi].gbest_val = bv;
+      sv[0] = 1.0;
+      si[0] = bi;
+    } else {
+      sv[0] = 0.0;
+    }
+  }
+  __syncthreads();
+  if (sv[0] != 0.0) {
+    const int64_t src = P.start0 + si[0]; // winner's local index -> global row
+    for (int d = threadIdx.x; d < D; d += NT)
+      a.gbx[int64_t(pi) * D + d] = a.pbx[src * D + d];
+  }
+}
+
+// Position/velocity of one coordinate: init (t = 0) or update (t >= 1).
+ZEUS_D ZEUS_FI void
+pso_coord(const PsoArgs& a, const ProblemDesc& P, int pi, int D, int64_t s, int d)
+{
+  const int64_t sl = s - P.start0;
+  const int64_t o = s * D + d;
+  if (a.iter == 0) {
+    const double vr = (P.upper - P.lower) * 0.1;
+    const double rx = rng::uniform_in(P.seed, P.stream, uint64_t(sl),
+                                      rng::pso_draw(D, 0, d, 0), P.lower, P.upper);
+    const double rv =
+      rng::uniform_in(P.seed, P.stream, uint64_t(sl), rng::pso_draw(D, 0, d, 1), -vr, vr);
+    a.X[o] = rx;
+    a.V[o] = rv;
+    a.pbx[o] = rx;
+  } else {
+    const double r1 = rng::uniform_in(P.seed, P.stream, uint64_t(sl),
+                                      rng::pso_draw(D, a.iter, d, 0), 0.0, 1.0);
+    const double r2 = rng::uniform_in(P.seed, P.stream, uint64_t(sl),
+                                      rng::pso_draw(D, a.iter, d, 1), 0.0, 1.0);
+    const double xi = a.X[o], vi = a.V[o], pb = a.pbx[o];
+    const double gb = a.gbx[int64_t(pi) * D + d];
+    const double nv = kPsoW * vi + kPsoC1 * r1 * (pb - xi) + kPsoC2 * r2 * (gb - xi);
+    a.V[o] = nv;
+    a.X[o] = xi + nv;
+  }
+}
+
+// Generic form: one lane per particle, objective evaluated by that lane.
+template <class F, int D, class Prov>
+__global__ __launch_bounds__(256) void
+pso_lane_kernel(PsoArgs a, Prov prov)
+{
+  const int pi = a.block_problem[blockIdx.x];
+  const ProblemDesc P = a.probs[pi];
+  if (a.iter > P.pso_iter)
+    return; // this problem runs fewer iterations (whole problem returns)
+  const int64_t sl = int64_t(blockIdx.x - P.block0) * 256 + threadIdx.x;
+  const bool live = sl < P.n_starts;
+  const int64_t s = P.start0 + sl;
+  double v = __builtin_nan("");
+  if (live) {
+    decltype(auto) f = prov.template get<F>(P);
+    for (int d = 0; d < D; ++d)
+      pso_coord(a, P, pi, D, s, d);
+    const double fv = lane_value<F, D>(f, a.X + s * D);
+    if (a.iter == 0) {
+      a.pbv[s] = fv;
+    } else if (fv < a.pbv[s]) {
+      a.pbv[s] = fv;
+      for (int d = 0; d < D; ++d)
+        a.pbx[s * D + d] = a.X[s * D + d];
+    }
+    v = a.pbv[s];
+  }
+  publish_best<256, 256, D>(a, pi, P, v, live ? sl : int64_t(1) << 62);
+}
+
+// Gaussian form: a workgroup owns 16 particles; X_tile (16 x D) times Cs
+// (D x D) on FP64 MFMA (v_mfma_f64_16x16x4f64), f_i = 0.5 x_i . (Cs x_i).
+// Wave w computes the 16 x 16 output tiles n = w, w + 4, ...
+typedef double mfma_acc_t __attribute__((ext_vector_type(4)));
+
+template <int D>
+__global__ __launch_bounds__(256) void
+pso_gaussian_mfma_kernel(PsoArgs a)
+{
+  constexpr int MT = 16;                 // particles per workgroup
+  constexpr int KP = (D + 3) / 4 * 4;    // K padded to the MFMA depth
+  constexpr int NTILE = (D + 15) / 16;   // 16-wide output tiles
+  constexpr int NP = NTILE * 16;
+  __shared__ double xs[MT][KP];
+  __shared__ double ys[MT][NP + 1];
+
+  const int pi = a.block_problem[blockIdx.x];
+  const ProblemDesc P = a.probs[pi];
+  if (a.iter > P.pso_iter)
+    return;
+  const int64_t base = int64_t(blockIdx.x - P.block0) * MT; // local index
+  const int tid = threadIdx.x;
+
+  // update / init coordinates and stage the tile
+  for (int e = tid; e < MT * KP; e += 256) {
+    const int r = e / KP, d = e % KP;
+    const int64_t sl = base + r;
+    double xv = 0.0;
+    if (sl < P.n_starts && d < D) {
+      const int64_t s = P.start0 + sl;
+      pso_coord(a, P, pi, D, s, d);
+      xv = a.X[s * D + d];
+    }
+    xs[r][d] = xv;
+  }
+  __syncthreads();
+
+  const double* Cs = P.aux;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int nt = w; nt < NTILE; nt += 4) {
+    const int col = nt * 16 + li;
+    mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < KP; k0 += 4) {
+      const int k = k0 + lk;
+      const double av = xs[li][k];                  // A[i = li][k]
+      const double bv = (k < D && col < D) ? Cs[k * D + col] : 0.0; // B[k][j = li]
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+    // D[i][j]: j = lane % 16, i = 4 * (lane / 16) + r
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      ys[4 * lk + r][col] = acc[r];
+  }
+  __syncthreads();
+
+  // f_i = 0.5 sum_j x_ij y_ij: 16 lanes per particle, fixed-order combine
+  const int row = tid >> 4, sub = tid & 15;
+  double t = 0.0;
+  for (int j = sub; j < D; j += 16)
+    t = ::fma(xs[row][j], ys[row][j], t);
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1)
+    t += __shfl_xor(t, off, 16);
+  const int64_t sl = base + row;
+  const bool live = sl < P.n_starts;
+  const int64_t s = P.start0 + sl;
+  double v = __builtin_nan("");
+  if (live && sub == 0) {
+    const double fv = 0.5 * t;
+    if (a.iter == 0) {
+      a.pbv[s] = fv;
+    } else if (fv < a.pbv[s]) {
+      a.pbv[s] = fv;
+      for (int d = 0; d < D; ++d)
+        a.pbx[s * D + d] = xs[row][d];
+    }
+    v = a.pbv[s];
+  }
+  publish_best<256, MT, D>(a, pi, P, v, (live && sub == 0) ? sl : int64_t(1) << 62);
+}
+
+} // namespace zeus::dev

@@ -1,0 +1,157 @@
+// summary.hpp -- control-block setup, objective preparation and the
+// per-problem result reduction.
+#pragma once
+
+#include "batch.hpp"
+#include "pso.hpp"
+
+#include <limits>
+
+namespace zeus::dev {
+
+// Zero every control block (a fill kernel rather than a memset node, so the
+// sequence can be captured into a graph and replayed).
+static __global__ void
+init_controls_kernel(Control* c, int n, double* gbx, int64_t n_gbx)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    Control z{};
+    z.best_idx = -1;
+    z.gbest_val = __builtin_huge_val();
+    c[i] = z;
+  }
+  for (int64_t k = i; k < n_gbx; k += int64_t(gridDim.x) * blockDim.x)
+    gbx[k] = 0.0;
+}
+
+// Cs = (C + C^T) / 2 for a Gaussian problem (same quadratic form).
+static __global__ void
+symmetrize_kernel(const double* C, double* Cs, int D)
+{
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < int64_t(D) * D) {
+    const int i = int(e / D), j = int(e % D);
+    Cs[e] = 0.5 * (C[int64_t(i) * D + j] + C[int64_t(j) * D + i]);
+  }
+}
+
+// One workgroup per problem: deterministic argmin of the start values (the
+// reference's launchReduction, bfgs_common.cuh:280-319, with NaN skipped
+// and ties to the lowest index) plus the status counts of
+// util.hpp:151-200 and the work counters used for the roofline.
+static __global__ __launch_bounds__(256) void
+summary_kernel(const ProblemDesc* probs, Control* ctl, StartOut out, double* best_x,
+               const int64_t* best_x_off)
+{
+  __shared__ double sv[4];
+  __shared__ int64_t si[4];
+  __shared__ int64_t sc[4][8];
+  const int pi = blockIdx.x;
+  const ProblemDesc P = probs[pi];
+  double v = __builtin_nan("");
+  int64_t idx = int64_t(1) << 62;
+  int64_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t k = threadIdx.x; k < P.n_starts; k += 256) {
+    const int64_t s = P.start0 + k;
+    const double f = out.fval[s];
+    if (better(f, k, v, idx)) {
+      v = f;
+      idx = k;
+    }
+    const int st = out.status[s];
+    cnt[0] += st == kStatusConverged;
+    cnt[1] += st == kStatusStopped;
+    cnt[2] += (st != kStatusConverged && st != kStatusStopped);
+    cnt[3] += st == kStatusNonFinite;
+    cnt[4] += out.iter[s];
+    cnt[5] += out.ls_evals[s];
+    cnt[6] += out.grads[s];
+    if (st == kStatusConverged) {
+      // Euclidean distance to the known minimum (reference src/util.cpp:11-41)
+      double e2 = 0.0;
+      for (int d = 0; d < P.dim; ++d) {
+        double m = 0.0;
+        if (P.min_kind == 1)
+          m = 1.0;
+        else if (P.min_kind == 2)
+          m = d == 1 ? -1.0 : 0.0;
+        const double df = out.x[s * P.dim + d] - m;
+        e2 += df * df;
+      }
+      cnt[7] += ::sqrt(e2) < 0.5;
+    }
+  }
+  block_argmin<256>(v, idx, sv, si);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    int64_t t = cnt[c];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1)
+      t += __shfl_xor(t, off, kWave);
+    if ((threadIdx.x & 63) == 0)
+      sc[threadIdx.x >> 6][c] = t;
+  }
+  __syncthreads();
+  __shared__ int64_t sbest;
+  if (threadIdx.x == 0) {
+    int64_t tot[8];
+    for (int c = 0; c < 8; ++c)
+      tot[c] = sc[0][c] + sc[1][c] + sc[2][c] + sc[3][c];
+    ctl[pi].actual = int32_t(tot[7]);
+    const int64_t b = P.n_starts > 0 ? (idx < P.n_starts ? idx : 0) : -1;
+    ctl[pi].best_idx = int32_t(b);
+    ctl[pi].claimed = int32_t(tot[0]);
+    ctl[pi].stopped = int32_t(tot[1]);
+    ctl[pi].surrendered = int32_t(tot[2]);
+    ctl[pi].nonfinite = int32_t(tot[3]);
+    ctl[pi].sum_iter = tot[4];
+    ctl[pi].sum_ls_evals = tot[5];
+    ctl[pi].sum_grads = tot[6];
+    if (b >= 0) {
+      const int64_t s = P.start0 + b;
+      ctl[pi].best_status = out.status[s];
+      ctl[pi].best_iter = out.iter[s];
+      ctl[pi].best_fval = out.fval[s];
+      ctl[pi].best_gnorm = out.gnorm[s];
+    }
+    sbest = b;
+  }
+  __syncthreads();
+  if (sbest >= 0)
+    for (int d = threadIdx.x; d < P.dim; d += 256)
+      best_x[best_x_off[pi] + d] = out.x[(P.start0 + sbest) * P.dim + d];
+}
+
+// Objective construction from a problem descriptor (built-in objectives).
+struct BuiltinProvider {
+  template <class F>
+  ZEUS_D ZEUS_FI F get(const ProblemDesc& P) const
+  {
+    F f{};
+    if constexpr (F::id == obj::kGaussian) {
+      f.C = P.params;
+    } else if constexpr (F::id == obj::kLogLikelihood) {
+      f.P = P.params;
+      f.n = int(P.n_params / 3);
+    } else if constexpr (F::id == obj::kMlp) {
+      f.P = P.params;
+      f.m = int(P.n_params / 2);
+    }
+    return f;
+  }
+};
+
+// A user functor uploaded once per call; kernels read it through a pointer
+// (no per-launch copy of the functor's members).
+template <class UF>
+struct UserProvider {
+  const UF* f;
+  template <class F>
+  ZEUS_D ZEUS_FI const UF& get(const ProblemDesc&) const
+  {
+    return *f;
+  }
+};
+
+} // namespace zeus::dev

@@ -1,0 +1,922 @@
+// zeus_c.hip -- implementation of the C ABI (include/zeus_c.h).
+//
+// Host orchestration of the multistart pipeline for the built-in
+// objectives: problems are grouped into classes (same objective and
+// dimension), each class runs PSO -> BFGS -> summary on one stream with no
+// host synchronisation in between, and only the per-problem summaries and
+// best coordinates come back to the host.  Restates the control flow of the
+// reference's impl::ZeusInternal (include/zeus.cuh:37-242),
+// pso::launch (include/pso.cuh:194-345) and bfgs::*::launch
+// (bfgs_sequential.cuh:384-589, bfgs_parallel.cuh:301-472).
+#include "zeus_c.h"
+
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace zeus::rt {
+
+const ClassOps*
+find_class(int objective, int dim)
+{
+  switch (objective) {
+    case obj::kRosenbrock: return ops_rosenbrock(dim);
+    case obj::kRastrigin: return ops_rastrigin(dim);
+    case obj::kAckley: return ops_ackley(dim);
+    case obj::kGaussian: return ops_gaussian(dim);
+    case obj::kLogLikelihood: return ops_loglik(dim);
+    case obj::kMlp: return ops_mlp(dim);
+    case obj::kSphere: return ops_sphere(dim);
+    default: return nullptr;
+  }
+}
+
+namespace {
+int g_fail_alloc_at = 0; // 1-based allocation number to fail, 0 = off
+int g_alloc_count = 0;
+int g_fail_launches = 0;
+} // namespace
+
+bool
+fault_take_launch()
+{
+  if (g_fail_launches > 0) {
+    --g_fail_launches;
+    return true;
+  }
+  return false;
+}
+
+} // namespace zeus::rt
+
+namespace {
+
+using namespace zeus;
+using zeus::dev::ClassArgs;
+using zeus::dev::Control;
+using zeus::dev::ProblemDesc;
+using zeus::dev::PsoArgs;
+using zeus::dev::StartOut;
+using zeus::rt::ClassOps;
+
+std::mutex g_mu;
+
+struct TuneKey {
+  int objective, dim, gradient;
+  bool operator<(const TuneKey& o) const
+  {
+    return std::tie(objective, dim, gradient) < std::tie(o.objective, o.dim, o.gradient);
+  }
+};
+struct TuneVal {
+  int form, threads;
+};
+
+struct DeviceState {
+  bool init = false;
+  int dev = 0;
+  int n_cu = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  std::map<TuneKey, TuneVal> tuned;
+};
+DeviceState g_ds;
+
+hipError_t
+dev_malloc(void** p, size_t bytes)
+{
+  ++rt::g_alloc_count;
+  if (rt::g_fail_alloc_at > 0 && rt::g_alloc_count == rt::g_fail_alloc_at) {
+    rt::g_fail_alloc_at = 0;
+    *p = nullptr;
+    return hipErrorOutOfMemory;
+  }
+  return hipMalloc(p, bytes);
+}
+
+zeus_status_t
+to_status(hipError_t e)
+{
+  if (e == hipSuccess)
+    return ZEUS_OK;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return ZEUS_E_ALLOC;
+  return ZEUS_E_LAUNCH;
+}
+
+zeus_status_t
+ensure_device()
+{
+  if (g_ds.init)
+    return ZEUS_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return ZEUS_E_NODEVICE;
+  if (hipGetDevice(&g_ds.dev) != hipSuccess)
+    return ZEUS_E_NODEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, g_ds.dev) != hipSuccess)
+    return ZEUS_E_NODEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return ZEUS_E_NODEVICE; // code objects are built for gfx950 only
+  g_ds.n_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&g_ds.stream, hipStreamNonBlocking) != hipSuccess)
+    return ZEUS_E_LAUNCH;
+  for (auto& e : g_ds.ev)
+    if (hipEventCreate(&e) != hipSuccess)
+      return ZEUS_E_LAUNCH;
+  g_ds.init = true;
+  return ZEUS_OK;
+}
+
+// Grow-only device arena; calls are host-synchronous, so a resize never
+// races with work in flight.
+zeus_status_t
+arena_reserve(size_t bytes)
+{
+  if (bytes <= g_ds.arena_bytes)
+    return ZEUS_OK;
+  if (g_ds.arena) {
+    (void)hipFree(g_ds.arena);
+    g_ds.arena = nullptr;
+    g_ds.arena_bytes = 0;
+  }
+  const size_t want = bytes + bytes / 4 + (1u << 20);
+  void* p = nullptr;
+  const hipError_t e = dev_malloc(&p, want);
+  if (e != hipSuccess)
+    return ZEUS_E_ALLOC;
+  g_ds.arena = p;
+  g_ds.arena_bytes = want;
+  return ZEUS_OK;
+}
+
+// Two-pass carving: with base == nullptr only measures.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  template <class T>
+  T* take(size_t n)
+  {
+    off = (off + 255) & ~size_t(255);
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += n * sizeof(T);
+    return p;
+  }
+};
+
+int
+min_kind_of(const char* name)
+{
+  if (!name)
+    return 0;
+  if (std::strcmp(name, "rosenbrock") == 0)
+    return 1;
+  if (std::strcmp(name, "goldstein_price") == 0)
+    return 2;
+  return 0;
+}
+
+zeus_status_t
+validate(const zeus_problem& p)
+{
+  if (p.dim <= 0 || p.n_starts < 0 || p.max_iter < 0 || p.n_starts > (int64_t(1) << 40))
+    return ZEUS_E_ARG;
+  switch (p.objective) {
+    case ZEUS_OBJ_ROSENBROCK:
+    case ZEUS_OBJ_RASTRIGIN:
+    case ZEUS_OBJ_ACKLEY:
+    case ZEUS_OBJ_SPHERE: break;
+    case ZEUS_OBJ_GAUSSIAN:
+      if (!p.params || p.n_params != int64_t(p.dim) * p.dim)
+        return ZEUS_E_ARG;
+      break;
+    case ZEUS_OBJ_LOGLIK:
+      if (p.dim != 4 || !p.params || p.n_params <= 0 || p.n_params % 3 != 0)
+        return ZEUS_E_ARG;
+      break;
+    case ZEUS_OBJ_MLP:
+      if (p.dim != 321 || !p.params || p.n_params <= 0 || p.n_params % 2 != 0)
+        return ZEUS_E_ARG;
+      break;
+    default: return ZEUS_E_ARG;
+  }
+  if (!rt::find_class(p.objective, p.dim))
+    return ZEUS_E_UNSUPPORTED;
+  return ZEUS_OK;
+}
+
+void
+default_result(zeus_result& r)
+{
+  std::memset(&r, 0, sizeof(r));
+  r.idx = -1;
+  r.status = ZEUS_UNSET;
+  r.fval = 37.0; // reference Result<D> defaults (util.hpp:36-54)
+}
+
+// Launch-shape choice per (objective, dim, gradient).  The per-thread form
+// is kept for small D, where one lane holds the whole state in registers;
+// the group form otherwise, with the largest offered workgroup for D >= 64.
+TuneVal
+choose_config(const ClassOps* ops, const zeus_options& opt)
+{
+  TuneVal v{ZEUS_FORM_GROUP, ops->group_nt[0]};
+  const int D = ops->dim;
+  int form = opt.form;
+  if (form == ZEUS_FORM_THREAD && !ops->thread_ok)
+    form = ZEUS_FORM_GROUP;
+  if (form == ZEUS_FORM_AUTO)
+    form = (ops->thread_ok && D <= 8) ? ZEUS_FORM_THREAD : ZEUS_FORM_GROUP;
+  v.form = form;
+  int nt = ops->group_nt[ops->n_group - 1];
+  if (D < 64)
+    nt = ops->group_nt[0];
+  if (opt.group_threads > 0)
+    for (int k = 0; k < ops->n_group; ++k)
+      if (ops->group_nt[k] == opt.group_threads)
+        nt = opt.group_threads;
+  v.threads = nt;
+  return v;
+}
+
+struct ClassPlan {
+  const ClassOps* ops = nullptr;
+  std::vector<int> probs; // indices into the caller's problem array
+  int64_t total = 0;
+  int pso_blocks = 0;
+  int max_pso_iter = -1;
+  TuneVal cfg{};
+  int grid = 0;
+  int64_t slab = 0; // doubles
+  // device carve
+  ProblemDesc* d_desc = nullptr;
+  int32_t* d_block_problem = nullptr;
+  double *X = nullptr, *V = nullptr, *pbx = nullptr, *pbv = nullptr, *gbx = nullptr;
+  double* part_val = nullptr;
+  int64_t* part_idx = nullptr;
+  StartOut out{};
+  double* aux = nullptr; // Gaussian Cs, one D x D block per problem
+  double* h_slab = nullptr;
+  int ctl0 = 0;          // first control block of the class
+  // host
+  std::vector<ProblemDesc> desc;
+  std::vector<int32_t> block_problem;
+};
+
+void
+carve_class(Carver& cv, ClassPlan& c)
+{
+  const int D = c.ops->dim;
+  const size_t n = size_t(c.total), np = c.probs.size(), nb = size_t(std::max(c.pso_blocks, 1));
+  c.d_desc = cv.take<ProblemDesc>(np);
+  c.d_block_problem = cv.take<int32_t>(nb);
+  const bool pso = c.max_pso_iter >= 0;
+  c.X = cv.take<double>(pso ? n * D : 1);
+  c.V = cv.take<double>(pso ? n * D : 1);
+  c.pbx = cv.take<double>(pso ? n * D : 1);
+  c.pbv = cv.take<double>(pso ? n : 1);
+  c.gbx = cv.take<double>(np * D);
+  c.part_val = cv.take<double>(nb);
+  c.part_idx = cv.take<int64_t>(nb);
+  c.out.status = cv.take<int32_t>(n);
+  c.out.iter = cv.take<int32_t>(n);
+  c.out.ls_evals = cv.take<int32_t>(n);
+  c.out.grads = cv.take<int32_t>(n);
+  c.out.fval = cv.take<double>(n);
+  c.out.gnorm = cv.take<double>(n);
+  c.out.x = cv.take<double>(n * D);
+  c.aux = cv.take<double>(c.ops->objective == obj::kGaussian ? np * D * D : 1);
+  c.h_slab = cv.take<double>(size_t(std::max<int64_t>(c.slab, 1)));
+}
+
+hipError_t
+launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
+{
+  const int D = c.ops->dim;
+  const int np = int(c.probs.size());
+  Control* ctl = ctl_all + c.ctl0;
+  hipError_t e = hipSuccess;
+  if (c.ops->objective == obj::kGaussian) {
+    for (int k = 0; k < np; ++k) {
+      const int64_t nn = int64_t(D) * D;
+      dev::symmetrize_kernel<<<int((nn + 255) / 256), 256, 0, st>>>(c.desc[k].params,
+                                                                    c.aux + k * nn, D);
+    }
+    if ((e = hipGetLastError()) != hipSuccess)
+      return e;
+  }
+  if (c.max_pso_iter >= 0 && c.pso_blocks > 0) {
+    PsoArgs pa{c.d_desc, np, 0, ctl, c.X, c.V, c.pbx, c.pbv, c.gbx,
+               c.part_val, c.part_idx, c.d_block_problem};
+    for (int it = 0; it <= c.max_pso_iter; ++it) {
+      pa.iter = it;
+      if ((e = c.ops->pso(pa, c.pso_blocks, st)) != hipSuccess)
+        return e;
+    }
+  }
+  return hipSuccess;
+}
+
+hipError_t
+launch_bfgs(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
+{
+  ClassArgs a{c.d_desc, int32_t(c.probs.size()), c.ops->dim, c.total, ctl_all + c.ctl0,
+              nullptr, c.out, c.h_slab, 0};
+  if (c.total == 0)
+    return hipSuccess;
+  if (c.cfg.form == ZEUS_FORM_THREAD)
+    return c.ops->bfgs_thread(a, gradient, st);
+  a.h_slab_stride = c.ops->group_slab_per_block(c.cfg.threads);
+  return c.ops->bfgs_group(a, gradient, c.cfg.threads, c.grid, st);
+}
+
+} // namespace
+
+extern "C" {
+
+zeus_status_t
+zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* opt_in,
+               zeus_result* results, double* best_x, const zeus_start_outputs* so,
+               void* stream)
+{
+  const auto t_start = std::chrono::steady_clock::now();
+  if (n_probs < 0 || (n_probs > 0 && (!probs || !results)))
+    return ZEUS_E_ARG;
+  zeus_options opt{};
+  if (opt_in)
+    opt = *opt_in;
+  if (opt.gradient != ZEUS_GRAD_ADJOINT && opt.gradient != ZEUS_GRAD_DUAL)
+    return ZEUS_E_ARG;
+  int64_t total_all = 0, dims_all = 0;
+  for (int i = 0; i < n_probs; ++i) {
+    const zeus_status_t v = validate(probs[i]);
+    if (v != ZEUS_OK)
+      return v;
+    total_all += probs[i].n_starts;
+    dims_all += probs[i].dim;
+  }
+  for (int i = 0; i < n_probs; ++i)
+    default_result(results[i]);
+  if (best_x)
+    std::fill(best_x, best_x + dims_all, 0.0);
+  if (total_all == 0)
+    return ZEUS_OK; // N == 0 -> Result{} (zeus.cuh:58-60)
+
+  std::lock_guard<std::mutex> lock(g_mu);
+  zeus_status_t rc = ensure_device();
+  if (rc != ZEUS_OK)
+    return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g_ds.stream;
+
+  // ---- plan: classes in order of first appearance ------------------------
+  std::vector<ClassPlan> classes;
+  std::vector<int> cls_of(n_probs, -1), local_of(n_probs, -1);
+  for (int i = 0; i < n_probs; ++i) {
+    const ClassOps* ops = rt::find_class(probs[i].objective, probs[i].dim);
+    int c = -1;
+    for (size_t k = 0; k < classes.size(); ++k)
+      if (classes[k].ops == ops)
+        c = int(k);
+    if (c < 0) {
+      classes.emplace_back();
+      classes.back().ops = ops;
+      c = int(classes.size()) - 1;
+    }
+    cls_of[i] = c;
+    local_of[i] = int(classes[c].probs.size());
+    classes[c].probs.push_back(i);
+  }
+  int ctl_total = 0;
+  for (auto& c : classes) {
+    c.ctl0 = ctl_total;
+    ctl_total += int(c.probs.size());
+    const int D = c.ops->dim;
+    c.cfg = choose_config(c.ops, opt);
+    g_ds.tuned[TuneKey{c.ops->objective, D, opt.gradient}] = c.cfg;
+    int64_t s0 = 0;
+    int b0 = 0;
+    for (int pi : c.probs) {
+      const zeus_problem& p = probs[pi];
+      ProblemDesc d{};
+      d.objective = p.objective;
+      d.dim = p.dim;
+      d.lower = p.lower;
+      d.upper = p.upper;
+      d.start0 = s0;
+      d.n_starts = p.n_starts;
+      d.required = p.required;
+      d.max_iter = p.max_iter;
+      d.pso_iter = (p.x0 != nullptr) ? -1 : p.pso_iter;
+      d.block0 = b0;
+      d.min_kind = min_kind_of(p.name);
+      d.seed = p.seed;
+      d.stream = p.stream;
+      d.tol = p.tol;
+      d.params = p.params;
+      d.n_params = p.n_params;
+      d.x0 = p.x0;
+      c.desc.push_back(d);
+      const int nb = int((p.n_starts + c.ops->pso_ppb - 1) / c.ops->pso_ppb);
+      for (int b = 0; b < nb; ++b)
+        c.block_problem.push_back(int32_t(c.desc.size() - 1));
+      b0 += nb;
+      s0 += p.n_starts;
+      c.max_pso_iter = std::max(c.max_pso_iter, d.pso_iter);
+    }
+    c.total = s0;
+    c.pso_blocks = b0;
+    if (c.cfg.form == ZEUS_FORM_THREAD) {
+      c.slab = c.ops->thread_slab_per_start * c.total;
+    } else {
+      const int occ = c.ops->group_blocks_per_cu(opt.gradient, c.cfg.threads);
+      c.grid = int(std::min<int64_t>(c.total, int64_t(g_ds.n_cu) * occ));
+      c.grid = std::max(c.grid, 1);
+      c.slab = c.ops->group_slab_per_block(c.cfg.threads) * c.grid;
+    }
+  }
+
+  // ---- device memory ------------------------------------------------------
+  auto layout = [&](Carver& cv, Control** ctl, double** bx, int64_t** bxo) {
+    for (auto& c : classes)
+      carve_class(cv, c);
+    *ctl = cv.take<Control>(size_t(ctl_total));
+    *bx = cv.take<double>(size_t(dims_all));
+    *bxo = cv.take<int64_t>(size_t(ctl_total));
+  };
+  Control* d_ctl;
+  double* d_bx;
+  int64_t* d_bxo;
+  {
+    Carver m{nullptr};
+    layout(m, &d_ctl, &d_bx, &d_bxo);
+    if ((rc = arena_reserve(m.off)) != ZEUS_OK) {
+      for (int i = 0; i < n_probs; ++i)
+        results[i].status = ZEUS_MALLOC;
+      return rc;
+    }
+    Carver cv{static_cast<char*>(g_ds.arena)};
+    layout(cv, &d_ctl, &d_bx, &d_bxo);
+  }
+  // problems that run PSO start their BFGS from pBestX
+  for (auto& c : classes)
+    for (auto& d : c.desc) {
+      if (d.x0 == nullptr && d.pso_iter >= 0)
+        d.x0 = c.pbx + d.start0 * d.dim;
+      if (c.ops->objective == obj::kGaussian)
+        d.aux = c.aux + (&d - c.desc.data()) * int64_t(d.dim) * d.dim;
+    }
+  // best-x offsets per control block (class order)
+  std::vector<int64_t> bxo(ctl_total);
+  std::vector<int64_t> bx_user(n_probs);
+  {
+    int64_t o = 0;
+    for (int i = 0; i < n_probs; ++i) {
+      bx_user[i] = o;
+      o += probs[i].dim;
+    }
+    for (int i = 0; i < n_probs; ++i)
+      bxo[classes[cls_of[i]].ctl0 + local_of[i]] = bx_user[i];
+  }
+
+  hipError_t e = hipSuccess;
+  auto fail = [&](hipError_t err) {
+    (void)hipStreamSynchronize(st);
+    const zeus_status_t s = to_status(err);
+    for (int i = 0; i < n_probs; ++i)
+      results[i].status = (s == ZEUS_E_ALLOC) ? ZEUS_MALLOC : ZEUS_KERNEL;
+    return s;
+  };
+  for (auto& c : classes) {
+    if ((e = hipMemcpyAsync(c.d_desc, c.desc.data(), sizeof(ProblemDesc) * c.desc.size(),
+                            hipMemcpyHostToDevice, st)) != hipSuccess)
+      return fail(e);
+    if (!c.block_problem.empty() &&
+        (e = hipMemcpyAsync(c.d_block_problem, c.block_problem.data(),
+                            sizeof(int32_t) * c.block_problem.size(), hipMemcpyHostToDevice,
+                            st)) != hipSuccess)
+      return fail(e);
+  }
+  if ((e = hipMemcpyAsync(d_bxo, bxo.data(), sizeof(int64_t) * bxo.size(),
+                          hipMemcpyHostToDevice, st)) != hipSuccess)
+    return fail(e);
+
+  // ---- launch sequence ----------------------------------------------------
+  dev::init_controls_kernel<<<(ctl_total + 255) / 256, 256, 0, st>>>(d_ctl, ctl_total, nullptr,
+                                                                     0);
+  for (auto& c : classes) {
+    const int64_t ng = int64_t(c.probs.size()) * c.ops->dim;
+    dev::init_controls_kernel<<<int((ng + 255) / 256), 256, 0, st>>>(nullptr, 0, c.gbx, ng);
+  }
+  if ((e = hipGetLastError()) != hipSuccess)
+    return fail(e);
+  (void)hipEventRecord(g_ds.ev[0], st);
+  for (auto& c : classes)
+    if ((e = launch_class(c, d_ctl, opt.gradient, st)) != hipSuccess)
+      return fail(e);
+  (void)hipEventRecord(g_ds.ev[1], st);
+  for (auto& c : classes)
+    if ((e = launch_bfgs(c, d_ctl, opt.gradient, st)) != hipSuccess)
+      return fail(e);
+  (void)hipEventRecord(g_ds.ev[2], st);
+  for (auto& c : classes) {
+    dev::summary_kernel<<<int(c.probs.size()), 256, 0, st>>>(c.d_desc, d_ctl + c.ctl0, c.out,
+                                                             d_bx, d_bxo + c.ctl0);
+  }
+  if ((e = hipGetLastError()) != hipSuccess)
+    return fail(e);
+
+  std::vector<Control> h_ctl(ctl_total);
+  if ((e = hipMemcpyAsync(h_ctl.data(), d_ctl, sizeof(Control) * ctl_total,
+                          hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return fail(e);
+  std::vector<double> h_bx(dims_all);
+  if ((e = hipMemcpyAsync(h_bx.data(), d_bx, sizeof(double) * dims_all, hipMemcpyDeviceToHost,
+                          st)) != hipSuccess)
+    return fail(e);
+  if ((e = hipStreamSynchronize(st)) != hipSuccess)
+    return fail(e);
+
+  float ms_pso = 0.f, ms_opt = 0.f;
+  (void)hipEventElapsedTime(&ms_pso, g_ds.ev[0], g_ds.ev[1]);
+  (void)hipEventElapsedTime(&ms_opt, g_ds.ev[1], g_ds.ev[2]);
+
+  // per-start outputs, scattered back into problem order
+  if (so) {
+    int64_t off_s = 0, off_x = 0;
+    std::vector<int64_t> offs(n_probs), offx(n_probs);
+    for (int i = 0; i < n_probs; ++i) {
+      offs[i] = off_s;
+      offx[i] = off_x;
+      off_s += probs[i].n_starts;
+      off_x += probs[i].n_starts * probs[i].dim;
+    }
+    for (auto& c : classes) {
+      const int D = c.ops->dim;
+      for (size_t k = 0; k < c.probs.size(); ++k) {
+        const int pi = c.probs[k];
+        const int64_t s0 = c.desc[k].start0, n = c.desc[k].n_starts;
+        if (n == 0)
+          continue;
+        auto cp = [&](void* dst, const void* src, size_t bytes) {
+          if (dst && e == hipSuccess)
+            e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+        };
+        cp(so->status ? so->status + offs[pi] : nullptr, c.out.status + s0, 4 * n);
+        cp(so->iter ? so->iter + offs[pi] : nullptr, c.out.iter + s0, 4 * n);
+        cp(so->ls_evals ? so->ls_evals + offs[pi] : nullptr, c.out.ls_evals + s0, 4 * n);
+        cp(so->grads ? so->grads + offs[pi] : nullptr, c.out.grads + s0, 4 * n);
+        cp(so->fval ? so->fval + offs[pi] : nullptr, c.out.fval + s0, 8 * n);
+        cp(so->gnorm ? so->gnorm + offs[pi] : nullptr, c.out.gnorm + s0, 8 * n);
+        cp(so->x ? so->x + offx[pi] : nullptr, c.out.x + s0 * D, 8 * n * D);
+      }
+    }
+    if (e != hipSuccess)
+      return fail(e);
+  }
+
+  // ---- results ------------------------------------------------------------
+  // best Result per problem: fetch its status/iter/fval/gnorm
+  const double ms_total =
+    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
+      .count();
+  for (int i = 0; i < n_probs; ++i) {
+    const ClassPlan& c = classes[cls_of[i]];
+    const Control& k = h_ctl[c.ctl0 + local_of[i]];
+    zeus_result& r = results[i];
+    if (probs[i].n_starts == 0)
+      continue;
+    r.idx = k.best_idx;
+    r.status = k.best_status;
+    r.iter = k.best_iter;
+    r.fval = k.best_fval;
+    r.gradient_norm = k.best_gnorm;
+    r.claimed = k.claimed;
+    r.actual = k.actual;
+    r.stopped = k.stopped;
+    r.surrendered = k.surrendered;
+    r.nonfinite = k.nonfinite;
+    r.sum_iter = k.sum_iter;
+    r.sum_ls_evals = k.sum_ls_evals;
+    r.sum_grads = k.sum_grads;
+    r.ms_pso = ms_pso;
+    r.ms_opt = ms_opt;
+    r.ms_total = ms_total;
+    if (best_x)
+      std::memcpy(best_x + bx_user[i], h_bx.data() + bx_user[i], sizeof(double) * probs[i].dim);
+  }
+  return ZEUS_OK;
+}
+
+} // extern "C"
+
+// ===========================================================================
+// Phase entry points, memory, random numbers, hooks.
+// ===========================================================================
+namespace {
+
+__global__ void
+argmin_kernel(const double* v, int64_t n, int64_t* out)
+{
+  __shared__ double sv[4];
+  __shared__ int64_t si[4];
+  double bv = __builtin_nan("");
+  int64_t bi = int64_t(1) << 62;
+  for (int64_t k = threadIdx.x; k < n; k += 256)
+    if (dev::better(v[k], k, bv, bi)) {
+      bv = v[k];
+      bi = k;
+    }
+  dev::block_argmin<256>(bv, bi, sv, si);
+  if (threadIdx.x == 0)
+    *out = n > 0 ? (bi < n ? bi : 0) : -1;
+}
+
+__global__ void
+philox_fill_kernel(double* out, int64_t n, uint32_t seed, uint32_t stream, uint64_t start,
+                   uint64_t draw0)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n)
+    out[i] = rng::uniform(seed, stream, start, draw0 + uint64_t(i));
+}
+
+} // namespace
+
+extern "C" {
+
+zeus_status_t
+zeus_pso_run(const zeus_problem* prob, double* pbest_x, double* pbest_val, double* gbest_x,
+             double* gbest_val, void* stream)
+{
+  if (!prob || !pbest_x || !pbest_val)
+    return ZEUS_E_ARG;
+  zeus_status_t rc = validate(*prob);
+  if (rc != ZEUS_OK)
+    return rc;
+  if (prob->pso_iter < 0)
+    return ZEUS_E_ARG;
+  if (prob->n_starts == 0)
+    return ZEUS_OK;
+  std::lock_guard<std::mutex> lock(g_mu);
+  if ((rc = ensure_device()) != ZEUS_OK)
+    return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g_ds.stream;
+  ClassPlan c;
+  c.ops = rt::find_class(prob->objective, prob->dim);
+  c.probs.push_back(0);
+  ProblemDesc d{};
+  d.objective = prob->objective;
+  d.dim = prob->dim;
+  d.lower = prob->lower;
+  d.upper = prob->upper;
+  d.n_starts = prob->n_starts;
+  d.pso_iter = prob->pso_iter;
+  d.seed = prob->seed;
+  d.stream = prob->stream;
+  d.params = prob->params;
+  d.n_params = prob->n_params;
+  c.desc.push_back(d);
+  c.pso_blocks = int((prob->n_starts + c.ops->pso_ppb - 1) / c.ops->pso_ppb);
+  c.block_problem.assign(c.pso_blocks, 0);
+  c.total = prob->n_starts;
+  c.max_pso_iter = prob->pso_iter;
+  Control* d_ctl;
+  auto layout = [&](Carver& cv) {
+    carve_class(cv, c);
+    d_ctl = cv.take<Control>(1);
+  };
+  {
+    Carver m{nullptr};
+    layout(m);
+    if ((rc = arena_reserve(m.off)) != ZEUS_OK)
+      return rc;
+    Carver cv{static_cast<char*>(g_ds.arena)};
+    layout(cv);
+  }
+  c.pbx = pbest_x;
+  c.pbv = pbest_val;
+  if (gbest_x)
+    c.gbx = gbest_x;
+  if (c.ops->objective == obj::kGaussian)
+    c.desc[0].aux = c.aux;
+  hipError_t e = hipMemcpyAsync(c.d_desc, c.desc.data(), sizeof(ProblemDesc),
+                                hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c.d_block_problem, c.block_problem.data(), 4 * c.block_problem.size(),
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    dev::init_controls_kernel<<<1, 256, 0, st>>>(d_ctl, 1, c.gbx, prob->dim);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = launch_class(c, d_ctl, 0, st);
+  if (e == hipSuccess && gbest_val)
+    e = hipMemcpyAsync(gbest_val, &d_ctl->gbest_val, sizeof(double), hipMemcpyDeviceToDevice,
+                       st);
+  const hipError_t e2 = hipStreamSynchronize(st);
+  return to_status(e != hipSuccess ? e : e2);
+}
+
+zeus_status_t
+zeus_eval(const zeus_problem* prob, const double* x, int64_t n, double* f, double* grad,
+          int32_t gradient, void* stream)
+{
+  if (!prob || n < 0 || (n > 0 && (!x || !f)))
+    return ZEUS_E_ARG;
+  if (gradient != ZEUS_GRAD_ADJOINT && gradient != ZEUS_GRAD_DUAL)
+    return ZEUS_E_ARG;
+  zeus_status_t rc = validate(*prob);
+  if (rc != ZEUS_OK)
+    return rc;
+  std::lock_guard<std::mutex> lock(g_mu);
+  if ((rc = ensure_device()) != ZEUS_OK)
+    return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g_ds.stream;
+  ProblemDesc d{};
+  d.objective = prob->objective;
+  d.dim = prob->dim;
+  d.params = prob->params;
+  d.n_params = prob->n_params;
+  const ClassOps* ops = rt::find_class(prob->objective, prob->dim);
+  hipError_t e = ops->eval(d, x, n, f, grad, gradient, st);
+  const hipError_t e2 = hipStreamSynchronize(st);
+  return to_status(e != hipSuccess ? e : e2);
+}
+
+zeus_status_t
+zeus_argmin(const double* values, int64_t n, int64_t* idx, void* stream)
+{
+  if (!idx || n < 0 || (n > 0 && !values))
+    return ZEUS_E_ARG;
+  if (n == 0) {
+    *idx = -1;
+    return ZEUS_OK;
+  }
+  std::lock_guard<std::mutex> lock(g_mu);
+  zeus_status_t rc = ensure_device();
+  if (rc != ZEUS_OK)
+    return rc;
+  if ((rc = arena_reserve(256)) != ZEUS_OK)
+    return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g_ds.stream;
+  int64_t* d_out = static_cast<int64_t*>(g_ds.arena);
+  if (rt::fault_take_launch())
+    return ZEUS_E_LAUNCH;
+  argmin_kernel<<<1, 256, 0, st>>>(values, n, d_out);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(idx, d_out, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+  const hipError_t e2 = hipStreamSynchronize(st);
+  return to_status(e != hipSuccess ? e : e2);
+}
+
+double
+zeus_philox_uniform(uint32_t seed, uint32_t stream, uint64_t start, uint64_t draw)
+{
+  return rng::uniform(seed, stream, start, draw);
+}
+
+zeus_status_t
+zeus_philox_fill(double* out, int64_t n, uint32_t seed, uint32_t stream, uint64_t start,
+                 uint64_t draw0, void* stream_handle)
+{
+  if (n < 0 || (n > 0 && !out))
+    return ZEUS_E_ARG;
+  if (n == 0)
+    return ZEUS_OK;
+  std::lock_guard<std::mutex> lock(g_mu);
+  zeus_status_t rc = ensure_device();
+  if (rc != ZEUS_OK)
+    return rc;
+  hipStream_t st = stream_handle ? static_cast<hipStream_t>(stream_handle) : g_ds.stream;
+  philox_fill_kernel<<<int((n + 255) / 256), 256, 0, st>>>(out, n, seed, stream, start, draw0);
+  hipError_t e = hipGetLastError();
+  const hipError_t e2 = hipStreamSynchronize(st);
+  return to_status(e != hipSuccess ? e : e2);
+}
+
+zeus_status_t
+zeus_alloc(void** ptr, uint64_t bytes)
+{
+  if (!ptr)
+    return ZEUS_E_ARG;
+  *ptr = nullptr;
+  if (bytes == 0)
+    return ZEUS_OK; // size 0 -> nullptr (cuda_buffer.hpp:253-260)
+  std::lock_guard<std::mutex> lock(g_mu);
+  zeus_status_t rc = ensure_device();
+  if (rc != ZEUS_OK)
+    return rc;
+  return dev_malloc(ptr, bytes) == hipSuccess ? ZEUS_OK : ZEUS_E_ALLOC;
+}
+
+zeus_status_t
+zeus_free(void* ptr)
+{
+  if (!ptr)
+    return ZEUS_OK;
+  return hipFree(ptr) == hipSuccess ? ZEUS_OK : ZEUS_E_LAUNCH;
+}
+
+zeus_status_t
+zeus_memcpy_h2d(void* dst, const void* src, uint64_t bytes)
+{
+  if (bytes == 0)
+    return ZEUS_OK;
+  if (!dst || !src)
+    return ZEUS_E_ARG;
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? ZEUS_OK
+                                                                          : ZEUS_E_LAUNCH;
+}
+
+zeus_status_t
+zeus_memcpy_d2h(void* dst, const void* src, uint64_t bytes)
+{
+  if (bytes == 0)
+    return ZEUS_OK;
+  if (!dst || !src)
+    return ZEUS_E_ARG;
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? ZEUS_OK
+                                                                          : ZEUS_E_LAUNCH;
+}
+
+zeus_status_t
+zeus_memcpy_d2d(void* dst, const void* src, uint64_t bytes)
+{
+  if (bytes == 0)
+    return ZEUS_OK;
+  if (!dst || !src)
+    return ZEUS_E_ARG;
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? ZEUS_OK
+                                                                            : ZEUS_E_LAUNCH;
+}
+
+zeus_status_t
+zeus_synchronize(void)
+{
+  return hipDeviceSynchronize() == hipSuccess ? ZEUS_OK : ZEUS_E_LAUNCH;
+}
+
+const char*
+zeus_status_string(int32_t status)
+{
+  switch (status) {
+    case ZEUS_UNSET: return "unset";
+    case ZEUS_SURRENDERED: return "surrendered";
+    case ZEUS_CONVERGED: return "converged";
+    case ZEUS_STOPPED: return "stopped early";
+    case ZEUS_MALLOC: return "device allocation failure";
+    case ZEUS_KERNEL: return "kernel launch or runtime error";
+    case ZEUS_NONFINITE: return "non-finite value";
+    case ZEUS_E_ARG: return "invalid argument";
+    case ZEUS_E_UNSUPPORTED: return "objective/dimension not instantiated";
+    case ZEUS_E_NODEVICE: return "no gfx950 device";
+    default: return "unknown";
+  }
+}
+
+zeus_status_t
+zeus_set_fault_hooks(int32_t fail_alloc_at, int32_t fail_launches)
+{
+  if (fail_alloc_at < 0 || fail_launches < 0)
+    return ZEUS_E_ARG;
+  std::lock_guard<std::mutex> lock(g_mu);
+  rt::g_alloc_count = 0;
+  rt::g_fail_alloc_at = fail_alloc_at;
+  rt::g_fail_launches = fail_launches;
+  // a forced allocation failure must reach the allocator: drop the arena
+  if (fail_alloc_at > 0 && g_ds.arena) {
+    (void)hipFree(g_ds.arena);
+    g_ds.arena = nullptr;
+    g_ds.arena_bytes = 0;
+  }
+  return ZEUS_OK;
+}
+
+zeus_status_t
+zeus_tuned_config(int32_t objective, int32_t dim, int32_t gradient, int32_t* form,
+                  int32_t* threads)
+{
+  if (!form || !threads)
+    return ZEUS_E_ARG;
+  std::lock_guard<std::mutex> lock(g_mu);
+  auto it = g_ds.tuned.find(TuneKey{objective, dim, gradient});
+  if (it == g_ds.tuned.end())
+    return ZEUS_E_ARG;
+  *form = it->second.form;
+  *threads = it->second.threads;
+  return ZEUS_OK;
+}
+
+} // extern "C"
