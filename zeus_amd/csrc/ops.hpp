@@ -54,7 +54,7 @@ struct Ops {
   }
 
   template <int NT>
-  static constexpr bool kHReg = dev::RowChunk<D, NT>::kPerThread <= 48;
+  static constexpr bool kHReg = dev::RowChunk<D, NT>::kPerThread <= 50;
 
   template <int NT, Gradient G>
   static auto group_kernel()
@@ -82,11 +82,23 @@ struct Ops {
     if constexpr (group_nt_count(D) > 1)
       if (nt == group_nt_at(D, 1))
         return group_nt<group_nt_at(D, 1)>(a, gradient, grid, s);
+    if constexpr (group_nt_count(D) > 2)
+      if (nt == group_nt_at(D, 2))
+        return group_nt<group_nt_at(D, 2)>(a, gradient, grid, s);
     return hipErrorInvalidConfiguration;
   }
 
   template <int NT>
   static int occ(int gradient)
+  {
+    static int cached[2] = {0, 0}; // per gradient; queried once per process
+    int& c = cached[gradient == 1];
+    if (c == 0)
+      c = occ_query<NT>(gradient);
+    return c;
+  }
+  template <int NT>
+  static int occ_query(int gradient)
   {
     int n = 0;
     hipError_t e;
@@ -106,20 +118,27 @@ struct Ops {
     if constexpr (group_nt_count(D) > 1)
       if (nt == group_nt_at(D, 1))
         return occ<group_nt_at(D, 1)>(gradient);
+    if constexpr (group_nt_count(D) > 2)
+      if (nt == group_nt_at(D, 2))
+        return occ<group_nt_at(D, 2)>(gradient);
     return 1;
   }
 
+  template <int NT>
+  static constexpr int64_t slab_doubles()
+  {
+    return kHReg<NT> ? 0 : int64_t(dev::RowChunk<D, NT>::kPerThread) * NT;
+  }
   static int64_t group_slab_per_block(int nt)
   {
     if (nt == group_nt_at(D, 0))
-      return kHReg<group_nt_at(D, 0)> ? 0
-                                      : int64_t(dev::RowChunk<D, group_nt_at(D, 0)>::kPerThread) *
-                                          group_nt_at(D, 0);
+      return slab_doubles<group_nt_at(D, 0)>();
     if constexpr (group_nt_count(D) > 1)
       if (nt == group_nt_at(D, 1))
-        return kHReg<group_nt_at(D, 1)>
-                 ? 0
-                 : int64_t(dev::RowChunk<D, group_nt_at(D, 1)>::kPerThread) * group_nt_at(D, 1);
+        return slab_doubles<group_nt_at(D, 1)>();
+    if constexpr (group_nt_count(D) > 2)
+      if (nt == group_nt_at(D, 2))
+        return slab_doubles<group_nt_at(D, 2)>();
     return 0;
   }
 

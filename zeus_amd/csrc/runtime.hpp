@@ -21,7 +21,7 @@ struct ClassOps {
   hipError_t (*bfgs_thread)(const dev::ClassArgs&, int gradient, hipStream_t) = nullptr;
   int64_t thread_slab_per_start = 0; // doubles of global H per start
   int n_group = 0;
-  int group_nt[4] = {0, 0, 0, 0};
+  int group_nt[4] = {0, 0, 0, 0}; // offered workgroup sizes
   hipError_t (*bfgs_group)(const dev::ClassArgs&, int gradient, int nt, int grid,
                            hipStream_t) = nullptr;
   int (*group_blocks_per_cu)(int gradient, int nt) = nullptr;
@@ -47,7 +47,7 @@ bool fault_take_launch();
 constexpr int
 group_nt_count(int D)
 {
-  return D <= 16 ? 1 : 2;
+  return D <= 16 ? 1 : (D <= 64 ? 2 : 3);
 }
 constexpr int
 group_nt_at(int D, int k)
@@ -56,7 +56,7 @@ group_nt_at(int D, int k)
     return 64;
   if (D <= 64)
     return k == 0 ? 64 : 256;
-  return k == 0 ? 256 : 1024;
+  return k == 0 ? 256 : (k == 1 ? 512 : 1024);
 }
 
 } // namespace zeus::rt

@@ -55,7 +55,17 @@ struct StartOut {
   double* x;        // n x D
   int32_t* ls_evals; // objective evaluations made by line searches
   int32_t* grads;    // gradient evaluations
+  int8_t* near;      // 1: converged within 0.5 (Euclidean) of the known minimum
 };
+
+// Coordinate d of the known minimum selected by ProblemDesc::min_kind
+// (reference src/util.cpp:11-41: rosenbrock -> ones, goldstein_price ->
+// (0, -1), anything else -> origin).
+ZEUS_HD ZEUS_FI double
+known_min(int kind, int d)
+{
+  return kind == 1 ? 1.0 : (kind == 2 && d == 1 ? -1.0 : 0.0);
+}
 
 struct ClassArgs {
   const ProblemDesc* probs;

@@ -35,11 +35,10 @@ struct GroupScratch {
 // the functor's adjoint on one lane, or by dual passes spread over lanes.
 template <class F, int D, int NT, Gradient G>
 struct GroupEvalLane {
-  F f;
   GroupScratch sc;
   int par = 0;
-  ZEUS_D void bind(const F& ff, const ProblemDesc&) { f = ff; }
-  ZEUS_D double value(const double* x, Block<NT>&)
+  ZEUS_D void bind(const ProblemDesc&) {}
+  ZEUS_D double value(const F& f, const double* x, Block<NT>&)
   {
     double* b = sc.scalar + par;
     par ^= 1;
@@ -48,7 +47,7 @@ struct GroupEvalLane {
     __syncthreads();
     return *b;
   }
-  ZEUS_D void gradient(const double* x, double* g, bool, Block<NT>&)
+  ZEUS_D void gradient(const F& f, const double* x, double* g, bool, Block<NT>&)
   {
     if constexpr (G == Gradient::Adjoint && HasAdjoint<F>) {
       if (threadIdx.x == 0)
@@ -78,15 +77,13 @@ template <int D, int NT, Gradient G>
 struct GroupEvalGaussian {
   using F = obj::Gaussian<D>;
   using Lay = RowChunk<D, NT>;
-  static constexpr bool kReg = Lay::kPerThread <= 40;
-  F f;
+  static constexpr bool kReg = Lay::kPerThread <= 50;
   DistMatrix<D, NT, kReg> C;
   const double* Cs = nullptr;
   GroupScratch sc;
 
-  ZEUS_D void bind(const F& ff, const ProblemDesc& P)
+  ZEUS_D void bind(const ProblemDesc& P)
   {
-    f = ff;
     Cs = P.aux;
     if constexpr (kReg)
       C.load(Cs);
@@ -104,15 +101,15 @@ struct GroupEvalGaussian {
           double acc = 0.0;
           for (int l = 0; l < Lay::L; ++l) {
             const int col = c * Lay::L + l;
-            if (col < D)
-              acc = ::fma(Cs[row * D + col], x[col], acc);
+            if (col < D) // Cs symmetric: read column-wise, lanes coalesce
+              acc = ::fma(Cs[col * D + row], x[col], acc);
           }
           sc.part[c * D + row] = acc;
         }
       }
     }
   }
-  ZEUS_D double value(const double* x, Block<NT>& blk)
+  ZEUS_D double value(const F&, const double* x, Block<NT>& blk)
   {
     partials(x);
     __syncthreads();
@@ -124,7 +121,8 @@ struct GroupEvalGaussian {
     }
     return 0.5 * blk.sum1(t);
   }
-  ZEUS_D void gradient(const double* x, double* g, bool at_last_value, Block<NT>& blk)
+  ZEUS_D void gradient(const F& f, const double* x, double* g, bool at_last_value,
+                       Block<NT>& blk)
   {
     if constexpr (G == Gradient::Dual) {
       group_dual_gradient<F, D, NT>(f, x, g);
@@ -147,11 +145,10 @@ struct GroupEvalGaussian {
 // sphere): lane i evaluates term i, the workgroup sums.
 template <class F, int D, int NT, Gradient G>
 struct GroupEvalSeparable {
-  F f;
   GroupScratch sc;
-  ZEUS_D void bind(const F& ff, const ProblemDesc&) { f = ff; }
+  ZEUS_D void bind(const ProblemDesc&) {}
 
-  ZEUS_D double value(const double* x, Block<NT>& blk)
+  ZEUS_D double value(const F&, const double* x, Block<NT>& blk)
   {
     if constexpr (F::id == obj::kAckley) {
       double v[2] = {0.0, 0.0};
@@ -188,7 +185,7 @@ struct GroupEvalSeparable {
       return x[i] * x[i];
     }
   }
-  ZEUS_D void gradient(const double* x, double* g, bool, Block<NT>& blk)
+  ZEUS_D void gradient(const F& f, const double* x, double* g, bool, Block<NT>& blk)
   {
     if constexpr (G == Gradient::Dual) {
       group_dual_gradient<F, D, NT>(f, x, g);
@@ -236,10 +233,9 @@ struct GroupEvalSeparable {
 template <int NT, Gradient G>
 struct GroupEvalLogLik {
   using F = obj::LogLikelihood;
-  F f;
   GroupScratch sc;
-  ZEUS_D void bind(const F& ff, const ProblemDesc&) { f = ff; }
-  ZEUS_D double value(const double* th, Block<NT>& blk)
+  ZEUS_D void bind(const ProblemDesc&) {}
+  ZEUS_D double value(const F& f, const double* th, Block<NT>& blk)
   {
     const double t0 = th[0], t1 = th[1], t2 = th[2], t3 = th[3];
     const double tt[4] = {t0, t1, t2, t3};
@@ -248,7 +244,7 @@ struct GroupEvalLogLik {
       t += f.bin_value(tt, i);
     return blk.sum1(t);
   }
-  ZEUS_D void gradient(const double* th, double* g, bool, Block<NT>& blk)
+  ZEUS_D void gradient(const F& f, const double* th, double* g, bool, Block<NT>& blk)
   {
     if constexpr (G == Gradient::Dual) {
       group_dual_gradient<F, 4, NT>(f, th, g);
@@ -276,32 +272,35 @@ struct GroupEvalLogLik {
   }
 };
 
-template <class F, int D, int NT, Gradient G>
+// Evaluator per objective type.  Specialisations key on the objective type
+// alone (its dimension parameter is a std::size_t; deducing it through an
+// int parameter would fail and silently select the one-lane fallback).
+template <class F, int NT, Gradient G>
 struct GroupEvalSelect {
-  using type = GroupEvalLane<F, D, NT, G>;
+  using type = GroupEvalLane<F, int(F::arity), NT, G>;
 };
-template <int D, int NT, Gradient G>
-struct GroupEvalSelect<obj::Gaussian<D>, D, NT, G> {
-  using type = GroupEvalGaussian<D, NT, G>;
+template <std::size_t D, int NT, Gradient G>
+struct GroupEvalSelect<obj::Gaussian<D>, NT, G> {
+  using type = GroupEvalGaussian<int(D), NT, G>;
 };
-template <int D, int NT, Gradient G>
-struct GroupEvalSelect<obj::Rosenbrock<D>, D, NT, G> {
-  using type = GroupEvalSeparable<obj::Rosenbrock<D>, D, NT, G>;
+template <std::size_t D, int NT, Gradient G>
+struct GroupEvalSelect<obj::Rosenbrock<D>, NT, G> {
+  using type = GroupEvalSeparable<obj::Rosenbrock<D>, int(D), NT, G>;
 };
-template <int D, int NT, Gradient G>
-struct GroupEvalSelect<obj::Rastrigin<D>, D, NT, G> {
-  using type = GroupEvalSeparable<obj::Rastrigin<D>, D, NT, G>;
+template <std::size_t D, int NT, Gradient G>
+struct GroupEvalSelect<obj::Rastrigin<D>, NT, G> {
+  using type = GroupEvalSeparable<obj::Rastrigin<D>, int(D), NT, G>;
 };
-template <int D, int NT, Gradient G>
-struct GroupEvalSelect<obj::Ackley<D>, D, NT, G> {
-  using type = GroupEvalSeparable<obj::Ackley<D>, D, NT, G>;
+template <std::size_t D, int NT, Gradient G>
+struct GroupEvalSelect<obj::Ackley<D>, NT, G> {
+  using type = GroupEvalSeparable<obj::Ackley<D>, int(D), NT, G>;
 };
-template <int D, int NT, Gradient G>
-struct GroupEvalSelect<obj::Sphere<D>, D, NT, G> {
-  using type = GroupEvalSeparable<obj::Sphere<D>, D, NT, G>;
+template <std::size_t D, int NT, Gradient G>
+struct GroupEvalSelect<obj::Sphere<D>, NT, G> {
+  using type = GroupEvalSeparable<obj::Sphere<D>, int(D), NT, G>;
 };
 template <int NT, Gradient G>
-struct GroupEvalSelect<obj::LogLikelihood, 4, NT, G> {
+struct GroupEvalSelect<obj::LogLikelihood, NT, G> {
   using type = GroupEvalLogLik<NT, G>;
 };
 
@@ -320,7 +319,7 @@ struct GroupVectors {
 
 template <class F, int D, int NT, Gradient G, bool HREG, class Eval>
 ZEUS_D void
-group_bfgs(GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval& ev,
+group_bfgs(const F& f, GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval& ev,
            Block<NT>& blk, const ProblemDesc& P, Control* ctl, int64_t s,
            const double* starts, const StartOut& out)
 {
@@ -343,8 +342,8 @@ group_bfgs(GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval& ev,
                                P.upper);
   }
   __syncthreads();
-  double bestVal = ev.value(L.x, blk);
-  ev.gradient(L.x, L.g, true, blk);
+  double bestVal = ev.value(f, L.x, blk);
+  ev.gradient(f, L.x, L.g, true, blk);
   __syncthreads();
   for (int i = tid; i < D; i += NT)
     L.p[i] = -L.g[i];
@@ -379,7 +378,7 @@ group_bfgs(GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval& ev,
       for (int j = tid; j < D; j += NT)
         L.xt[j] = ::fma(alpha, L.p[j], L.x[j]);
       __syncthreads();
-      f1 = ev.value(L.xt, blk);
+      f1 = ev.value(f, L.xt, blk);
       ++ls_evals;
       if (f1 <= ::fma(kArmijoC1 * alpha, ddir, bestVal) && f1 < bestVal) {
         accepted = true;
@@ -392,10 +391,10 @@ group_bfgs(GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval& ev,
       for (int j = tid; j < D; j += NT)
         L.xt[j] = ::fma(alpha, L.p[j], L.x[j]);
       __syncthreads();
-      fnew = ev.value(L.xt, blk);
+      fnew = ev.value(f, L.xt, blk);
       ++ls_evals;
     }
-    ev.gradient(L.xt, L.gn, true, blk);
+    ev.gradient(f, L.xt, L.gn, true, blk);
     ++grads;
     __syncthreads();
 
@@ -483,6 +482,15 @@ group_bfgs(GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval& ev,
       gg = ::fma(L.g[i], L.g[i], gg);
     gnorm = ::sqrt(blk.sum1(gg));
   }
+  double e2 = 0.0;
+  if (status == kStatusConverged) {
+    double t = 0.0;
+    for (int d = tid; d < D; d += NT) {
+      const double df = L.x[d] - known_min(P.min_kind, d);
+      t = ::fma(df, df, t);
+    }
+    e2 = blk.sum1(t);
+  }
   if (tid == 0) {
     out.status[s] = status;
     out.iter[s] = iter;
@@ -490,6 +498,7 @@ group_bfgs(GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval& ev,
     out.gnorm[s] = gnorm;
     out.ls_evals[s] = ls_evals;
     out.grads[s] = grads;
+    out.near[s] = status == kStatusConverged && ::sqrt(e2) < 0.5;
   }
   for (int d = tid; d < D; d += NT)
     out.x[s * D + d] = L.x[d];
@@ -502,7 +511,8 @@ bfgs_group_kernel(ClassArgs a, Prov prov)
 {
   __shared__ GroupVectors<D, NT> L;
   Block<NT> blk{L.red};
-  using Eval = typename GroupEvalSelect<F, D, NT, G>::type;
+  using Eval = typename GroupEvalSelect<F, NT, G>::type;
+  static_assert(int(F::arity) == D, "objective arity and kernel dimension differ");
   Eval ev;
   ev.sc = GroupScratch{L.part, L.stash, L.scalar};
   DistMatrix<D, NT, HREG> H;
@@ -513,10 +523,11 @@ bfgs_group_kernel(ClassArgs a, Prov prov)
     const int pi = find_problem(a.probs, a.n_probs, s);
     const ProblemDesc P = a.probs[pi];
     if (pi != bound) {
-      ev.bind(prov.template get<F>(P), P);
+      ev.bind(P);
       bound = pi;
     }
-    group_bfgs<F, D, NT, G, HREG>(L, H, ev, blk, P, a.ctl + pi, s, a.starts, a.out);
+    decltype(auto) f = prov.template get<F>(P);
+    group_bfgs<F, D, NT, G, HREG>(f, L, H, ev, blk, P, a.ctl + pi, s, a.starts, a.out);
   }
 }
 

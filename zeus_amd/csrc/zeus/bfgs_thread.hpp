@@ -220,6 +220,16 @@ thread_bfgs(const F& f, const ProblemDesc& P, Control* ctl, int64_t s,
     out.x[s * D + d] = x[d];
   out.ls_evals[s] = ls_evals;
   out.grads[s] = grads;
+  int8_t near = 0;
+  if (status == kStatusConverged) {
+    double e2 = 0.0;
+    for (int d = 0; d < D; ++d) {
+      const double df = x[d] - known_min(P.min_kind, d);
+      e2 += df * df;
+    }
+    near = ::sqrt(e2) < 0.5;
+  }
+  out.near[s] = near;
 }
 
 template <class F, int D, Gradient G, class Prov>
@@ -231,7 +241,7 @@ bfgs_thread_kernel(ClassArgs a, Prov prov)
     return;
   const int pi = find_problem(a.probs, a.n_probs, s);
   const ProblemDesc P = a.probs[pi];
-  const F f = prov.template get<F>(P);
+  decltype(auto) f = prov.template get<F>(P);
   if constexpr (D <= kThreadRegMaxD) {
     RegPackedH<D> H;
     thread_bfgs<F, D, G>(f, P, a.ctl + pi, s, a.starts, a.out, H);

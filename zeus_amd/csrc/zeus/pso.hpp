@@ -105,17 +105,21 @@ publish_best(const PsoArgs& a, int pi, const ProblemDesc& P, double v, int64_t i
   if (!last)
     return;
   __threadfence();
-  if (threadIdx.x == 0) {
-    double bv = a.part_val[P.block0];
-    int64_t bi = a.part_idx[P.block0];
-    for (int b = 1; b < nblk; ++b) {
-      const double ov = a.part_val[P.block0 + b];
-      const int64_t oi = a.part_idx[P.block0 + b];
-      if (better(ov, oi, bv, bi)) {
-        bv = ov;
-        bi = oi;
-      }
+  // fold the partials with the whole workgroup; `better` is a total order,
+  // so the result does not depend on the folding order
+  double bv = __builtin_nan("");
+  int64_t bi = int64_t(1) << 62;
+  for (int b = threadIdx.x; b < nblk; b += NT) {
+    const double ov = a.part_val[P.block0 + b];
+    const int64_t oi = a.part_idx[P.block0 + b];
+    if (better(ov, oi, bv, bi)) {
+      bv = ov;
+      bi = oi;
     }
+  }
+  __syncthreads(); // sv/si are reused by the second argmin
+  block_argmin<NT>(bv, bi, sv, si);
+  if (threadIdx.x == 0) {
     a.ctl[pi].pso_arrive = 0u;
     if (bv < a.ctl[pi].gbest_val) {
       a.ctl[pi].gbest_val = bv;
@@ -204,8 +208,11 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   constexpr int KP = (D + 3) / 4 * 4;    // K padded to the MFMA depth
   constexpr int NTILE = (D + 15) / 16;   // 16-wide output tiles
   constexpr int NP = NTILE * 16;
+  constexpr int KC = 16;                 // rows of Cs staged in LDS per chunk
+  constexpr int TPW = (NTILE + 3) / 4;   // output tiles per wavefront
   __shared__ double xs[MT][KP];
   __shared__ double ys[MT][NP + 1];
+  __shared__ double cs[KC][NP + 1];
 
   const int pi = a.block_problem[blockIdx.x];
   const ProblemDesc P = a.probs[pi];
@@ -228,22 +235,46 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   }
   __syncthreads();
 
+  // Y = X_tile Cs: Cs streamed through LDS in KC-row chunks (coalesced
+  // global reads, shared by the four wavefronts); wave w owns the output
+  // tiles w, w + 4, ... (operands: A[i][k] on lane i + 16k, B[k][j] on lane
+  // j + 16k; result D[i][j] in register i / 4 of lane j + 16 (i % 4)).
   const double* Cs = P.aux;
   const int lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  for (int nt = w; nt < NTILE; nt += 4) {
-    const int col = nt * 16 + li;
-    mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < KP; k0 += 4) {
-      const int k = k0 + lk;
-      const double av = xs[li][k];                  // A[i = li][k]
-      const double bv = (k < D && col < D) ? Cs[k * D + col] : 0.0; // B[k][j = li]
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-    }
-    // D[i][j]: j = lane % 16, i = 4 * (lane / 16) + r
+  mfma_acc_t acc[TPW];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      ys[4 * lk + r][col] = acc[r];
+  for (int t = 0; t < TPW; ++t)
+    acc[t] = mfma_acc_t{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < KP; k0 += KC) {
+    for (int e = tid; e < KC * NP; e += 256) {
+      const int r = e / NP, c = e % NP, k = k0 + r;
+      cs[r][c] = (k < D && c < D) ? Cs[k * D + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 4) {
+      if (k0 + kk < KP) {
+        const double av = xs[li][k0 + kk + lk];
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+          const int nt = w + 4 * t;
+          if (nt < NTILE)
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, cs[kk + lk][nt * 16 + li], acc[t],
+                                                         0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int nt = w + 4 * t;
+    if (nt < NTILE) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        ys[4 * r + lk][nt * 16 + li] = acc[t][r];
+    }
   }
   __syncthreads();
 

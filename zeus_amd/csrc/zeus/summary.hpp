@@ -67,20 +67,7 @@ summary_kernel(const ProblemDesc* probs, Control* ctl, StartOut out, double* bes
     cnt[4] += out.iter[s];
     cnt[5] += out.ls_evals[s];
     cnt[6] += out.grads[s];
-    if (st == kStatusConverged) {
-      // Euclidean distance to the known minimum (reference src/util.cpp:11-41)
-      double e2 = 0.0;
-      for (int d = 0; d < P.dim; ++d) {
-        double m = 0.0;
-        if (P.min_kind == 1)
-          m = 1.0;
-        else if (P.min_kind == 2)
-          m = d == 1 ? -1.0 : 0.0;
-        const double df = out.x[s * P.dim + d] - m;
-        e2 += df * df;
-      }
-      cnt[7] += ::sqrt(e2) < 0.5;
-    }
+    cnt[7] += out.near[s];
   }
   block_argmin<256>(v, idx, sv, si);
 #pragma unroll

@@ -69,11 +69,14 @@ using zeus::rt::ClassOps;
 
 std::mutex g_mu;
 
+// Autotuner cache key: objective, dimension, gradient method and the class's
+// start count rounded down to a power of two.
 struct TuneKey {
-  int objective, dim, gradient;
+  int objective, dim, gradient, bucket;
   bool operator<(const TuneKey& o) const
   {
-    return std::tie(objective, dim, gradient) < std::tie(o.objective, o.dim, o.gradient);
+    return std::tie(objective, dim, gradient, bucket) <
+           std::tie(o.objective, o.dim, o.gradient, o.bucket);
   }
 };
 struct TuneVal {
@@ -85,7 +88,7 @@ struct DeviceState {
   int dev = 0;
   int n_cu = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {};
+  hipEvent_t ev[6] = {};
   void* arena = nullptr;
   size_t arena_bytes = 0;
   std::map<TuneKey, TuneVal> tuned;
@@ -225,29 +228,34 @@ default_result(zeus_result& r)
   r.fval = 37.0; // reference Result<D> defaults (util.hpp:36-54)
 }
 
-// Launch-shape choice per (objective, dim, gradient).  The per-thread form
-// is kept for small D, where one lane holds the whole state in registers;
-// the group form otherwise, with the largest offered workgroup for D >= 64.
-TuneVal
-choose_config(const ClassOps* ops, const zeus_options& opt)
+// Launch shapes the autotuner may choose between for a class: the
+// per-thread form (where offered) and every offered workgroup size of the
+// per-workgroup form, unless the caller fixed the form or the size.
+std::vector<TuneVal>
+candidates(const ClassOps* ops, const zeus_options& opt)
 {
-  TuneVal v{ZEUS_FORM_GROUP, ops->group_nt[0]};
-  const int D = ops->dim;
-  int form = opt.form;
-  if (form == ZEUS_FORM_THREAD && !ops->thread_ok)
-    form = ZEUS_FORM_GROUP;
-  if (form == ZEUS_FORM_AUTO)
-    form = (ops->thread_ok && D <= 8) ? ZEUS_FORM_THREAD : ZEUS_FORM_GROUP;
-  v.form = form;
-  int nt = ops->group_nt[ops->n_group - 1];
-  if (D < 64)
-    nt = ops->group_nt[0];
-  if (opt.group_threads > 0)
+  std::vector<TuneVal> v;
+  const bool want_thread = opt.form == ZEUS_FORM_THREAD || opt.form == ZEUS_FORM_AUTO;
+  const bool want_group =
+    opt.form == ZEUS_FORM_GROUP || opt.form == ZEUS_FORM_AUTO || !ops->thread_ok;
+  if (want_thread && ops->thread_ok)
+    v.push_back({ZEUS_FORM_THREAD, 0});
+  if (want_group || v.empty())
     for (int k = 0; k < ops->n_group; ++k)
-      if (ops->group_nt[k] == opt.group_threads)
-        nt = opt.group_threads;
-  v.threads = nt;
+      if (opt.group_threads <= 0 || ops->group_nt[k] == opt.group_threads)
+        v.push_back({ZEUS_FORM_GROUP, ops->group_nt[k]});
+  if (v.empty()) // requested size not offered: fall back to the largest
+    v.push_back({ZEUS_FORM_GROUP, ops->group_nt[ops->n_group - 1]});
   return v;
+}
+
+int
+size_bucket(int64_t n)
+{
+  int b = 0;
+  while ((int64_t(1) << (b + 1)) <= n)
+    ++b;
+  return b;
 }
 
 struct ClassPlan {
@@ -257,6 +265,8 @@ struct ClassPlan {
   int pso_blocks = 0;
   int max_pso_iter = -1;
   TuneVal cfg{};
+  std::vector<TuneVal> cands; // > 1 entry: time them and keep the fastest
+  TuneKey key{};
   int grid = 0;
   int64_t slab = 0; // doubles
   // device carve
@@ -293,6 +303,7 @@ carve_class(Carver& cv, ClassPlan& c)
   c.out.iter = cv.take<int32_t>(n);
   c.out.ls_evals = cv.take<int32_t>(n);
   c.out.grads = cv.take<int32_t>(n);
+  c.out.near = cv.take<int8_t>(n);
   c.out.fval = cv.take<double>(n);
   c.out.gnorm = cv.take<double>(n);
   c.out.x = cv.take<double>(n * D);
@@ -328,17 +339,70 @@ launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
   return hipSuccess;
 }
 
+// Persistent grid of the group form: as many workgroups as can be resident.
+int
+group_grid(const ClassPlan& c, const TuneVal& v, int gradient)
+{
+  const int occ = c.ops->group_blocks_per_cu(gradient, v.threads);
+  const int g = int(std::min<int64_t>(c.total, int64_t(g_ds.n_cu) * occ));
+  return std::max(g, 1);
+}
+
+int64_t
+slab_doubles(const ClassPlan& c, const TuneVal& v, int gradient)
+{
+  if (v.form == ZEUS_FORM_THREAD)
+    return c.ops->thread_slab_per_start * c.total;
+  return c.ops->group_slab_per_block(v.threads) * group_grid(c, v, gradient);
+}
+
 hipError_t
-launch_bfgs(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
+launch_bfgs_cfg(ClassPlan& c, const TuneVal& v, Control* ctl_all, int gradient, hipStream_t st)
 {
   ClassArgs a{c.d_desc, int32_t(c.probs.size()), c.ops->dim, c.total, ctl_all + c.ctl0,
               nullptr, c.out, c.h_slab, 0};
   if (c.total == 0)
     return hipSuccess;
-  if (c.cfg.form == ZEUS_FORM_THREAD)
+  if (v.form == ZEUS_FORM_THREAD)
     return c.ops->bfgs_thread(a, gradient, st);
-  a.h_slab_stride = c.ops->group_slab_per_block(c.cfg.threads);
-  return c.ops->bfgs_group(a, gradient, c.cfg.threads, c.grid, st);
+  a.h_slab_stride = c.ops->group_slab_per_block(v.threads);
+  return c.ops->bfgs_group(a, gradient, v.threads, group_grid(c, v, gradient), st);
+}
+
+// BFGS phase of a class.  With several candidate shapes (first call for this
+// key), each runs on the real inputs from the same PSO result and control
+// state; the fastest is cached and its results are the ones kept.
+hipError_t
+launch_bfgs(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
+{
+  if (c.cands.size() <= 1)
+    return launch_bfgs_cfg(c, c.cfg, ctl_all, gradient, st);
+  const int np = int(c.probs.size());
+  float best_ms = 0.f;
+  size_t best = 0;
+  hipError_t e = hipSuccess;
+  for (size_t k = 0; k < c.cands.size(); ++k) {
+    dev::init_controls_kernel<<<(np + 255) / 256, 256, 0, st>>>(ctl_all + c.ctl0, np, nullptr, 0);
+    (void)hipEventRecord(g_ds.ev[4], st);
+    if ((e = launch_bfgs_cfg(c, c.cands[k], ctl_all, gradient, st)) != hipSuccess)
+      return e;
+    (void)hipEventRecord(g_ds.ev[5], st);
+    if ((e = hipEventSynchronize(g_ds.ev[5])) != hipSuccess)
+      return e;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, g_ds.ev[4], g_ds.ev[5]);
+    if (k == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = k;
+    }
+  }
+  c.cfg = c.cands[best];
+  g_ds.tuned[c.key] = c.cfg;
+  if (best + 1 != c.cands.size()) {
+    dev::init_controls_kernel<<<(np + 255) / 256, 256, 0, st>>>(ctl_all + c.ctl0, np, nullptr, 0);
+    e = launch_bfgs_cfg(c, c.cfg, ctl_all, gradient, st);
+  }
+  return e;
 }
 
 } // namespace
@@ -401,9 +465,6 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   for (auto& c : classes) {
     c.ctl0 = ctl_total;
     ctl_total += int(c.probs.size());
-    const int D = c.ops->dim;
-    c.cfg = choose_config(c.ops, opt);
-    g_ds.tuned[TuneKey{c.ops->objective, D, opt.gradient}] = c.cfg;
     int64_t s0 = 0;
     int b0 = 0;
     for (int pi : c.probs) {
@@ -436,14 +497,21 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     }
     c.total = s0;
     c.pso_blocks = b0;
-    if (c.cfg.form == ZEUS_FORM_THREAD) {
-      c.slab = c.ops->thread_slab_per_start * c.total;
-    } else {
-      const int occ = c.ops->group_blocks_per_cu(opt.gradient, c.cfg.threads);
-      c.grid = int(std::min<int64_t>(c.total, int64_t(g_ds.n_cu) * occ));
-      c.grid = std::max(c.grid, 1);
-      c.slab = c.ops->group_slab_per_block(c.cfg.threads) * c.grid;
+    c.key = TuneKey{c.ops->objective, c.ops->dim, opt.gradient, size_bucket(c.total)};
+    c.cands = candidates(c.ops, opt);
+    if (c.cands.size() > 1) {
+      auto it = g_ds.tuned.find(c.key);
+      if (it != g_ds.tuned.end())
+        for (const auto& v : c.cands)
+          if (v.form == it->second.form && v.threads == it->second.threads)
+            c.cands = {v};
     }
+    c.cfg = c.cands.front();
+    if (c.cands.size() == 1)
+      g_ds.tuned[c.key] = c.cfg;
+    c.slab = 0;
+    for (const auto& v : c.cands)
+      c.slab = std::max(c.slab, slab_doubles(c, v, opt.gradient));
   }
 
   // ---- device memory ------------------------------------------------------
@@ -911,11 +979,15 @@ zeus_tuned_config(int32_t objective, int32_t dim, int32_t gradient, int32_t* for
   if (!form || !threads)
     return ZEUS_E_ARG;
   std::lock_guard<std::mutex> lock(g_mu);
-  auto it = g_ds.tuned.find(TuneKey{objective, dim, gradient});
-  if (it == g_ds.tuned.end())
+  // the largest size bucket tuned for this (objective, dim, gradient)
+  const TuneVal* found = nullptr;
+  for (const auto& kv : g_ds.tuned)
+    if (kv.first.objective == objective && kv.first.dim == dim && kv.first.gradient == gradient)
+      found = &kv.second;
+  if (!found)
     return ZEUS_E_ARG;
-  *form = it->second.form;
-  *threads = it->second.threads;
+  *form = found->form;
+  *threads = found->threads;
   return ZEUS_OK;
 }
 
