@@ -5,6 +5,8 @@
 Every .hip file under zeus_amd/csrc is compiled with hipcc for gfx950 only
 into zeus_amd/_build/, then linked into zeus_amd/lib/libzeus_amd.so.
 Without --force, objects newer than every source and header are reused.
+The C++ front-end test programs under tests/cpp/*.hip are then compiled
+against the library into tests/cpp/bin/ (with an rpath to zeus_amd/lib).
 """
 from __future__ import annotations
 
@@ -22,6 +24,8 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libzeus_amd.so")
+CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+CPP_TEST_BIN = os.path.join(CPP_TESTS, "bin")
 
 ARCH = "gfx950"
 # -ffp-contract=off: the fused multiply-adds of the method are written out as
@@ -89,7 +93,33 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = True) ->
         os.replace(tmp, LIB)
         if verbose:
             print(f"[zeus_amd.build] linked {os.path.relpath(LIB, ROOT)}", flush=True)
+    build_cpp_tests(force=force or bool(todo), verbose=verbose)
     return LIB
+
+
+def build_cpp_tests(force: bool = False, verbose: bool = True) -> list[str]:
+    """Compile each tests/cpp/*.hip into an executable linked to the library."""
+    srcs = sorted(glob.glob(os.path.join(CPP_TESTS, "*.hip")))
+    if not srcs:
+        return []
+    os.makedirs(CPP_TEST_BIN, exist_ok=True)
+    newest = max([os.path.getmtime(h) for h in _headers()] + [os.path.getmtime(LIB)])
+    rpath = "-Wl,-rpath,$ORIGIN/" + os.path.relpath(LIBDIR, CPP_TEST_BIN)
+    out = []
+    for s in srcs:
+        exe = os.path.join(CPP_TEST_BIN, os.path.basename(s)[:-4])
+        out.append(exe)
+        if (not force and os.path.exists(exe) and os.path.getmtime(exe) >= os.path.getmtime(s)
+                and os.path.getmtime(exe) >= newest):
+            continue
+        cmd = [hipcc(), *HIPFLAGS, s, "-o", exe, f"-L{LIBDIR}", "-lzeus_amd", rpath]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            err = "\n".join(l for l in p.stderr.splitlines() if "packed-fp32-ops" not in l)
+            raise RuntimeError(f"compilation of {os.path.relpath(s, ROOT)} failed:\n{err}")
+        if verbose:
+            print(f"[zeus_amd.build] compiled {os.path.relpath(exe, ROOT)}", flush=True)
+    return out
 
 
 def main() -> None:

@@ -1,0 +1,243 @@
+// zeus.hpp -- C++ front end: the reference's zeus::Zeus API for user functors.
+//
+//   auto r = zeus::Zeus(f, lower, upper, N, MAX_ITER, PSO_ITER,
+//                       requiredConverged, fun_name, tolerance, seed, run,
+//                       parallel = true, prng = PRNGType::XORWOW,
+//                       trajectory_file = {}, nzerosteps = 0);
+//
+// Same signature, defaults, Result<D> fields and status codes as the
+// reference (include/zeus.cuh:341-375).  F is any reference-style callable
+// (operator() over std::array<T, D> for T = double and dual::DualNumber);
+// when F also has `double value_and_gradient(const double*, double*) const`
+// the BFGS uses that adjoint instead of D dual passes (override with the
+// trailing `gradient` argument).  Compile the calling file with hipcc for
+// gfx950 and link libzeus_amd.so: the PSO, BFGS and summary kernel
+// templates are instantiated here for F; device memory and copies go
+// through the library's C ABI (zeus_c.h).  F's bytes are uploaded once per
+// call and read by the kernels through a pointer, so a Matrix member is
+// never deep-copied per launch.
+#pragma once
+
+#include "zeus_c.h"
+
+#include "batch.hpp"
+#include "bfgs_group.hpp"
+#include "bfgs_thread.hpp"
+#include "config.hpp"
+#include "dual.hpp"
+#include "exception.hpp"
+#include "hip_buffer.hpp"
+#include "matrix.hpp"
+#include "objectives.hpp"
+#include "pso.hpp"
+#include "result.hpp"
+#include "summary.hpp"
+#include "traits.hpp"
+#include "util_types.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <vector>
+
+namespace zeus {
+
+namespace detail {
+
+inline void
+hip_check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess)
+    throw HipError(e, what);
+}
+
+inline void
+abi_check(int rc, const char* what)
+{
+  if (rc != ZEUS_OK)
+    throw HipError(hip_error_of(rc), what);
+}
+
+// Workgroup size of the per-workgroup form for user functors: the objective
+// is evaluated by one lane and its dual passes spread over the lanes.
+template <int D>
+constexpr int
+front_group_threads()
+{
+  return D <= 64 ? 64 : (D <= 256 ? 256 : 512);
+}
+
+inline int
+min_kind_of(std::string_view name)
+{
+  if (name == "rosenbrock")
+    return 1;
+  if (name == "goldstein_price")
+    return 2;
+  return 0;
+}
+
+template <class F, int D, Gradient G>
+Result<D>
+run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO_ITER,
+    int required, std::string_view fun_name, double tol, int seed, bool parallel)
+{
+  Result<D> res;
+  if (N == 0)
+    return res; // reference zeus.cuh:58-60
+  if (MAX_ITER < 0)
+    throw std::invalid_argument("MAX_ITER must be >= 0");
+  try {
+    using namespace dev;
+    const bool pso = PSO_ITER >= 0;
+    const int64_t n = int64_t(N);
+    const int nblk = int((n + 255) / 256);
+
+    HipBuffer<unsigned char> fbuf(sizeof(F));
+    abi_check(zeus_memcpy_h2d(fbuf.data(), &f, sizeof(F)), "functor upload");
+    const UserProvider<F> prov{reinterpret_cast<const F*>(fbuf.data())};
+
+    HipBuffer<double> X(pso ? n * D : 0), V(pso ? n * D : 0), pbx(pso ? n * D : 0),
+      pbv(pso ? n : 0), gbx(D), part_val(nblk);
+    HipBuffer<int64_t> part_idx(nblk), bxo(1);
+    HipBuffer<int32_t> bprob(nblk);
+    std::vector<int32_t> zeros(nblk, 0);
+    abi_check(zeus_memcpy_h2d(bprob.data(), zeros.data(), 4 * nblk), "upload");
+    const int64_t zero64 = 0;
+    abi_check(zeus_memcpy_h2d(bxo.data(), &zero64, 8), "upload");
+
+    ProblemDesc P{};
+    P.dim = D;
+    P.lower = lower;
+    P.upper = upper;
+    P.start0 = 0;
+    P.n_starts = n;
+    P.required = required;
+    P.max_iter = MAX_ITER;
+    P.pso_iter = PSO_ITER;
+    P.block0 = 0;
+    P.min_kind = min_kind_of(fun_name);
+    P.seed = uint32_t(seed);
+    P.stream = 0;
+    P.tol = tol;
+    P.x0 = pso ? pbx.data() : nullptr;
+    HipBuffer<ProblemDesc> dP(1);
+    abi_check(zeus_memcpy_h2d(dP.data(), &P, sizeof(P)), "upload");
+    HipBuffer<Control> ctl(1);
+
+    HipBuffer<int32_t> st(n), it(n), ls(n), gr(n);
+    HipBuffer<double> fv(n), gn(n), xo(n * D), bx(D);
+    HipBuffer<int8_t> near(n);
+    StartOut out{st.data(), it.data(), fv.data(), gn.data(), xo.data(), ls.data(), gr.data(),
+                 near.data()};
+
+    constexpr int NT = front_group_threads<D>();
+    constexpr bool kHReg = RowChunk<D, NT>::kPerThread <= 50;
+    int grid = 1;
+    int64_t slab = 0, stride = 0;
+    if (parallel) {
+      int dev = 0, n_cu = 1, occ = 1;
+      hip_check(hipGetDevice(&dev), "hipGetDevice");
+      hip_check(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev),
+                "hipDeviceGetAttribute");
+      hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &occ,
+                  reinterpret_cast<const void*>(
+                    &bfgs_group_kernel<F, D, NT, G, kHReg, UserProvider<F>>),
+                  NT, 0),
+                "occupancy");
+      grid = int(std::max<int64_t>(1, std::min<int64_t>(n, int64_t(n_cu) * std::max(occ, 1))));
+      if (!kHReg) {
+        stride = int64_t(RowChunk<D, NT>::kPerThread) * NT;
+        slab = stride * grid;
+      }
+    } else if (D > kThreadRegMaxD) {
+      slab = n * (int64_t(D) * (D + 1) / 2);
+    }
+    HipBuffer<double> hslab(slab);
+
+    hipStream_t s = nullptr;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    init_controls_kernel<<<1, 256, 0, s>>>(ctl.data(), 1, gbx.data(), D);
+    if (pso) {
+      PsoArgs pa{dP.data(), 1, 0, ctl.data(), X.data(), V.data(), pbx.data(), pbv.data(),
+                 gbx.data(), part_val.data(), part_idx.data(), bprob.data()};
+      for (int t = 0; t <= PSO_ITER; ++t) {
+        pa.iter = t;
+        pso_lane_kernel<F, D, UserProvider<F>><<<nblk, 256, 0, s>>>(pa, prov);
+      }
+    }
+    hip_check(hipGetLastError(), "PSO launch");
+    hip_check(hipEventRecord(e0, s), "event");
+    ClassArgs a{dP.data(), 1, D, n, ctl.data(), nullptr, out, hslab.data(), stride};
+    if (parallel)
+      bfgs_group_kernel<F, D, NT, G, kHReg, UserProvider<F>><<<grid, NT, 0, s>>>(a, prov);
+    else
+      bfgs_thread_kernel<F, D, G, UserProvider<F>><<<nblk, 256, 0, s>>>(a, prov);
+    hip_check(hipGetLastError(), "BFGS launch");
+    hip_check(hipEventRecord(e1, s), "event");
+    summary_kernel<<<1, 256, 0, s>>>(dP.data(), ctl.data(), out, bx.data(), bxo.data());
+    hip_check(hipGetLastError(), "summary launch");
+    hip_check(hipStreamSynchronize(s), "BFGS runtime");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+
+    Control k{};
+    abi_check(zeus_memcpy_d2h(&k, ctl.data(), sizeof(k)), "result copy");
+    abi_check(zeus_memcpy_d2h(res.coordinates, bx.data(), sizeof(double) * D), "result copy");
+    res.idx = k.best_idx;
+    res.status = k.best_status;
+    res.iter = k.best_iter;
+    res.fval = k.best_fval;
+    res.gradientNorm = k.best_gnorm;
+    res.c.claimed = k.claimed;
+    res.c.actual = k.actual;
+    res.c.stopped = k.stopped;
+    res.c.surrendered = k.surrendered;
+    res.ms_opt = ms;
+  } catch (const HipError& e) {
+    res = Result<D>{};
+    res.status = (e.code() == hipErrorOutOfMemory || e.code() == hipErrorMemoryAllocation)
+                   ? kStatusMalloc
+                   : kStatusKernel;
+  }
+  return res;
+}
+
+} // namespace detail
+
+template <typename Function, std::size_t ZEUS_DIM = FnTraits<Function>::arity>
+  requires ZeusObjective<Function, ZEUS_DIM>
+auto
+Zeus(Function const& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO_ITER,
+     int requiredConverged, std::string_view fun_name, double tolerance, int seed, int run,
+     bool parallel = true, PRNGType prng = PRNGType::XORWOW,
+     std::string_view trajectory_file = {}, int nzerosteps = 0,
+     Gradient gradient = Gradient::Adjoint) -> Result<ZEUS_DIM>
+{
+  (void)run;        // labels the call only
+  (void)prng;       // every PRNGType draws from the counter-based Philox stream
+  (void)nzerosteps; // inert: the line search never returns a zero step
+  if (!trajectory_file.empty())
+    throw std::invalid_argument("zeus::Zeus: trajectory recording is not part of this build");
+  constexpr int D = int(ZEUS_DIM);
+  if constexpr (HasAdjoint<Function>)
+    if (gradient == Gradient::Adjoint)
+      return detail::run<Function, D, Gradient::Adjoint>(f, lower, upper, N, MAX_ITER, PSO_ITER,
+                                                         requiredConverged, fun_name, tolerance,
+                                                         seed, parallel);
+  return detail::run<Function, D, Gradient::Dual>(f, lower, upper, N, MAX_ITER, PSO_ITER,
+                                                  requiredConverged, fun_name, tolerance, seed,
+                                                  parallel);
+}
+
+} // namespace zeus
