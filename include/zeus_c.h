@@ -59,9 +59,15 @@ typedef int32_t zeus_status_t;
 #define ZEUS_GRAD_ADJOINT 0
 #define ZEUS_GRAD_DUAL 1
 
-#define ZEUS_FORM_AUTO 0   /* autotuner's choice                          */
+#define ZEUS_FORM_AUTO 0   /* autotuner's choice among all forms          */
 #define ZEUS_FORM_THREAD 1 /* one lane per start (reference sequential)   */
-#define ZEUS_FORM_GROUP 2  /* one workgroup per start (reference parallel)*/
+#define ZEUS_FORM_GROUP 2  /* workgroup forms (reference parallel): one
+                              workgroup per start, or -- Gaussian, adjoint
+                              gradient, unless group_threads is set -- the
+                              tile form; the autotuner picks              */
+#define ZEUS_FORM_TILE 3   /* one workgroup per tile of 16 starts, the
+                              objective evaluated for the tile on FP64 MFMA
+                              (Gaussian with dim >= 16, adjoint gradient)  */
 
 /* One optimization problem: the arguments of zeus::Zeus for a built-in
    objective.  Replaces the template parameters/arguments of
@@ -182,8 +188,9 @@ const char* zeus_status_string(int32_t status);
    without exhausting or crashing the device. */
 zeus_status_t zeus_set_fault_hooks(int32_t fail_alloc_at, int32_t fail_launches);
 
-/* Workgroup size / form the autotuner picked for (objective, dim, gradient)
-   after the last zeus_run_batch, for inspection: *form, *threads. */
+/* Form and workgroup size the last zeus_run_batch used for (objective, dim,
+   gradient) -- the autotuner's choice unless the options fixed them -- for
+   inspection: *form (ZEUS_FORM_*), *threads. */
 zeus_status_t zeus_tuned_config(int32_t objective, int32_t dim,
                                 int32_t gradient, int32_t* form,
                                 int32_t* threads);

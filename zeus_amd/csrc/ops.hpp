@@ -23,13 +23,15 @@ template <class F, int D>
 struct Ops {
   static constexpr bool kMfmaPso = F::id == obj::kGaussian && D >= 16;
   static constexpr bool kThreadOk = F::id != obj::kMlp && D <= 64;
+  // tile form: Gaussian classes whose PSO already runs in tiles of 16
+  static constexpr bool kTileOk = kMfmaPso;
 
   static hipError_t pso(const dev::PsoArgs& a, int nb, hipStream_t s)
   {
     if (fault_take_launch())
       return hipErrorLaunchFailure;
     if constexpr (kMfmaPso)
-      dev::pso_gaussian_mfma_kernel<D><<<nb, 256, 0, s>>>(a);
+      dev::pso_gaussian_mfma_kernel<D><<<nb, dev::kPsoTileThreads, 0, s>>>(a);
     else
       dev::pso_lane_kernel<F, D, dev::BuiltinProvider><<<nb, 256, 0, s>>>(a, {});
     return hipGetLastError();
@@ -142,6 +144,33 @@ struct Ops {
     return 0;
   }
 
+  static hipError_t bfgs_tile(const dev::TileArgs& a, int grid, hipStream_t s)
+  {
+    if constexpr (!kTileOk) {
+      return hipErrorInvalidConfiguration;
+    } else {
+      if (fault_take_launch())
+        return hipErrorLaunchFailure;
+      dev::bfgs_tile_kernel<D><<<grid, dev::kTileThreads, 0, s>>>(a);
+      return hipGetLastError();
+    }
+  }
+  static int tile_blocks_per_cu()
+  {
+    if constexpr (!kTileOk) {
+      return 1;
+    } else {
+      static int cached = 0;
+      if (cached == 0) {
+        int n = 0;
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &n, reinterpret_cast<const void*>(&dev::bfgs_tile_kernel<D>), dev::kTileThreads, 0);
+        cached = (e == hipSuccess && n > 0) ? n : 1;
+      }
+      return cached;
+    }
+  }
+
   static hipError_t eval(const dev::ProblemDesc& P, const double* x, int64_t n, double* f,
                          double* g, int gradient, hipStream_t s)
   {
@@ -163,7 +192,7 @@ struct Ops {
       ClassOps o;
       o.objective = F::id;
       o.dim = D;
-      o.pso_ppb = kMfmaPso ? 16 : 256;
+      o.pso_ppb = kMfmaPso ? dev::kPsoTile : 256;
       o.pso = &pso;
       o.thread_ok = kThreadOk;
       o.bfgs_thread = &bfgs_thread;
@@ -175,6 +204,10 @@ struct Ops {
       o.group_blocks_per_cu = &group_blocks_per_cu;
       o.group_slab_per_block = &group_slab_per_block;
       o.eval = &eval;
+      o.tile_ok = kTileOk;
+      o.bfgs_tile = &bfgs_tile;
+      o.tile_blocks_per_cu = &tile_blocks_per_cu;
+      o.tile_slot_doubles = kTileOk ? dev::TileShape<D>::kSlotDoubles : 0;
       return o;
     }();
     return &t;

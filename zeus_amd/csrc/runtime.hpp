@@ -4,6 +4,7 @@
 #include "zeus/batch.hpp"
 #include "zeus/bfgs_group.hpp"
 #include "zeus/bfgs_thread.hpp"
+#include "zeus/bfgs_tile.hpp"
 #include "zeus/pso.hpp"
 #include "zeus/summary.hpp"
 
@@ -26,6 +27,11 @@ struct ClassOps {
                            hipStream_t) = nullptr;
   int (*group_blocks_per_cu)(int gradient, int nt) = nullptr;
   int64_t (*group_slab_per_block)(int nt) = nullptr; // doubles, 0 if resident
+  // tile form (16 starts per workgroup, objective on MFMA; adjoint only)
+  bool tile_ok = false;
+  hipError_t (*bfgs_tile)(const dev::TileArgs&, int grid, hipStream_t) = nullptr;
+  int (*tile_blocks_per_cu)() = nullptr;
+  int64_t tile_slot_doubles = 0; // doubles of global memory per start slot
   hipError_t (*eval)(const dev::ProblemDesc&, const double* x, int64_t n, double* f,
                      double* g, int gradient, hipStream_t) = nullptr;
 };

@@ -1,0 +1,487 @@
+// bfgs_tile.hpp -- tile form of the BFGS for the Gaussian objective: one
+// 1024-thread workgroup advances a tile of 16 starts of one problem in
+// lock-step rounds.
+//
+// Round structure.  Every start of the tile submits one point per round (its
+// first point, a line-search trial x + alpha p, or the unconditional point
+// after 20 rejected trials).  The 16 points Xt (D x 16, LDS) are evaluated
+// together: G = Cs Xt on FP64 MFMA (v_mfma_f64_16x16x4f64), Cs = (C + C^T)/2
+// streamed from L2 as A operands, f_j = 0.5 Xt_j . G_j, and G_j is the
+// adjoint gradient of start j at its point.  The product is split into
+// (16-row block, half of the k range) units spread evenly over the 16
+// wavefronts (and so over the four SIMDs); the two k halves land in two LDS
+// partial buffers that are added in a fixed order.  Wavefront w then runs
+// start w's BFGS logic for the round -- line-search decision, update,
+// convergence test -- with DPP wavefront reductions only, and submits its
+// next point.  Two workgroup barriers per round.
+//
+// Method.  The reference's per-start BFGS (include/bfgs_sequential.cuh:
+// 46-221, utils.cuh:245-287, 376-426) with the same line search, update
+// skip rule, statuses and counters as the other forms.  The inverse Hessian
+// is kept as H = B + sum_j (s_j a_j^T + a_j s_j^T): the rank-2 pairs of the
+// update H' = H + s a^T + a s^T (a = (beta/2) s - rho u, u = H y), which is
+// the reference's update expanded; B = I until the pair list is full, then
+// the pairs are folded into an explicit B (global memory) and the list
+// restarts.  H g needs 2k dot products and 2k axpys instead of a pass over
+// D^2 entries.  Sums run in a fixed order (results are deterministic), but
+// in another order than the serial oracle: parity is to tolerance.
+#pragma once
+
+#include "batch.hpp"
+#include "block.hpp"
+#include "philox.hpp"
+#include "pso.hpp"
+
+namespace zeus::dev {
+
+constexpr int kTileStarts = 16; // starts per tile = MFMA N dimension
+// Workgroup size of the tile form: 8 wavefronts, each running the logic of
+// two starts (a 256-register budget per lane: the Cs operands of a product
+// unit are all in flight at once without spilling).
+constexpr int kTileThreads = 512;
+constexpr int kTileWaves = kTileThreads / kWave;
+constexpr int kSlotsPerWave = kTileStarts / kTileWaves;
+static_assert(kTileStarts % kTileWaves == 0, "whole slots per wavefront");
+
+template <int D>
+struct TileShape {
+  static constexpr int MB = (D + 15) / 16; // 16-row blocks of Cs
+  static constexpr int KB = (D + 3) / 4;   // 4-deep k steps
+  static constexpr int KP = KB * 4;        // padded depth
+  static constexpr int EL = (D + kWave - 1) / kWave; // rows per lane in start logic
+  static constexpr int XS = kTileStarts + 1;         // padded LDS row stride
+  static constexpr int KB1 = (KB + 1) / 2;           // k steps of the first half
+  static constexpr int U = 2 * MB;                   // (row block, k half) units
+  static constexpr int kHist = 32; // rank-2 pairs kept before folding into B
+  static constexpr int64_t kSlotDoubles = int64_t(kHist) * 2 * D + int64_t(D) * D;
+};
+
+struct TileArgs {
+  const ProblemDesc* probs;
+  const int32_t* tile_problem; // tile -> problem (tiles of a problem are consecutive)
+  int64_t n_tiles;
+  Control* ctl;
+  StartOut out;
+  double* slot_mem; // per (workgroup, start slot): kHist pairs, then B (D x D)
+};
+
+// Per-start state of a slot (LDS; read and written by its wavefront only).
+struct TileSlot {
+  int64_t s;   // global start index
+  int phase;   // 0 idle, 1 first point, 2 trial, 3 unconditional point
+  int trial, iter, grads, ls_evals, k;
+  int base;    // explicit B in use
+  double best, alpha, ddir;
+};
+
+template <int D>
+struct TileLds {
+  using S = TileShape<D>;
+  double xt[S::KP * S::XS]; // submitted points, [row][slot]
+  double gt[2 * D * S::XS]; // Cs Xt by k half, [half][row][slot]
+  double fpart[S::U * kTileStarts];
+  double x[kTileStarts * D], g[kTileStarts * D], p[kTileStarts * D];
+  int live[kTileStarts];
+  TileSlot slot[kTileStarts];
+  ProblemDesc prob; // the tile's problem
+};
+
+template <int EL>
+ZEUS_D ZEUS_FI double
+tile_dot(const double (&a)[EL], const double (&b)[EL])
+{
+  double t = 0.0;
+#pragma unroll
+  for (int e = 0; e < EL; ++e)
+    t = ::fma(a[e], b[e], t);
+  return wave_sum(t);
+}
+
+// Logic of start slot w for one round, run by one wavefront; `f` is the
+// value at the slot's submitted point.
+template <int D>
+ZEUS_D void
+tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
+               const StartOut& out, double* mem, double f)
+{
+  using S = TileShape<D>;
+  constexpr int EL = S::EL;
+  TileSlot& st = L.slot[w];
+  const int lane = threadIdx.x % kWave;
+  double* X = L.x + w * D;
+  double* G = L.g + w * D;
+  double* Pd = L.p + w * D;
+  double* hist = mem;
+  double* B = mem + int64_t(S::kHist) * 2 * D;
+  int row[EL];
+  bool ok[EL];
+#pragma unroll
+  for (int e = 0; e < EL; ++e) {
+    row[e] = lane + e * kWave;
+    ok[e] = row[e] < D;
+  }
+  auto load = [&](const double* v, double (&r)[EL]) {
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      r[e] = ok[e] ? v[row[e]] : 0.0;
+  };
+  auto load_col = [&](const double* m, double (&r)[EL]) { // [row][slot] array
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      r[e] = ok[e] ? m[row[e] * S::XS + w] : 0.0;
+  };
+  auto load_grad = [&](double (&r)[EL]) { // G column: k halves in fixed order
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      r[e] = ok[e] ? L.gt[row[e] * S::XS + w] + L.gt[(D + row[e]) * S::XS + w] : 0.0;
+  };
+  auto submit = [&](int phase) {
+    double x[EL], p[EL];
+    load(X, x);
+    load(Pd, p);
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      if (ok[e])
+        L.xt[row[e] * S::XS + w] = ::fma(st.alpha, p[e], x[e]);
+    st.phase = phase;
+  };
+  auto finish = [&](int status, double fval, double gnorm) {
+    double e2 = 0.0;
+    if (status == kStatusConverged) {
+      double x[EL], d[EL];
+      load(X, x);
+#pragma unroll
+      for (int e = 0; e < EL; ++e)
+        d[e] = ok[e] ? x[e] - known_min(P.min_kind, row[e]) : 0.0;
+      e2 = tile_dot<EL>(d, d);
+    }
+    if (lane == 0) {
+      out.status[st.s] = status;
+      out.iter[st.s] = st.iter;
+      out.fval[st.s] = fval;
+      out.gnorm[st.s] = gnorm;
+      out.ls_evals[st.s] = st.ls_evals;
+      out.grads[st.s] = st.grads;
+      out.near[st.s] = status == kStatusConverged && ::sqrt(e2) < 0.5;
+    }
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      if (ok[e])
+        out.x[st.s * D + row[e]] = X[row[e]];
+    st.phase = 0;
+  };
+  auto gnorm_now = [&]() {
+    double g[EL];
+    load(G, g);
+    return ::sqrt(tile_dot<EL>(g, g));
+  };
+  // top of a BFGS iteration: MAX_ITER, stop flag, first trial
+  auto begin_iteration = [&]() {
+    if (st.iter >= P.max_iter) {
+      finish(kStatusSurrendered, st.best, gnorm_now());
+      return;
+    }
+    if (load_flag(&ctl->stop_flag) != 0) {
+      finish(kStatusStopped, st.best, gnorm_now());
+      return;
+    }
+    double g[EL], p[EL];
+    load(G, g);
+    load(Pd, p);
+    st.ddir = tile_dot<EL>(g, p);
+    st.alpha = 1.0;
+    st.trial = 0;
+    submit(2);
+  };
+
+  if (st.phase == 1) { // first point: value and gradient
+    double g[EL];
+    load_grad(g);
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      if (ok[e]) {
+        G[row[e]] = g[e];
+        Pd[row[e]] = -g[e];
+      }
+    st.best = f;
+    st.grads = 1;
+    begin_iteration();
+    return;
+  }
+  if (st.phase == 2) {
+    ++st.ls_evals;
+    const bool accept = f <= ::fma(kArmijoC1 * st.alpha, st.ddir, st.best) && f < st.best;
+    if (!accept) {
+      st.alpha *= 0.5;
+      if (++st.trial == kLineSearchTrials)
+        submit(3);
+      else
+        submit(2);
+      return;
+    }
+  } else { // phase 3: point after the last rejected trial
+    ++st.ls_evals;
+  }
+
+  // ---- end of the iteration: gradient at the new point and the update ----
+  // (short live ranges: vectors are re-read from LDS where they are needed)
+  const double fnew = f;
+  ++st.grads;
+  const bool acc = fnew < st.best;
+  double gn[EL], v[EL];
+  load_grad(gn);
+  // v = H gn, H = B + sum_j (s_j a_j^T + a_j s_j^T)
+  if (st.base) {
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      v[e] = 0.0;
+#pragma unroll 1
+    for (int c = 0; c < D; ++c) {
+      const double gc = L.gt[c * S::XS + w] + L.gt[(D + c) * S::XS + w];
+#pragma unroll
+      for (int e = 0; e < EL; ++e)
+        if (ok[e])
+          v[e] = ::fma(B[int64_t(c) * D + row[e]], gc, v[e]);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      v[e] = gn[e];
+  }
+#pragma unroll 1
+  for (int j = 0; j < st.k; ++j) {
+    double sj[EL], aj[EL];
+    load(hist + int64_t(j) * 2 * D, sj);
+    load(hist + int64_t(j) * 2 * D + D, aj);
+    const double cj = tile_dot<EL>(aj, gn);
+    const double dj = tile_dot<EL>(sj, gn);
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      v[e] = ::fma(sj[e], cj, ::fma(aj[e], dj, v[e]));
+  }
+  // s = xt - x, y = gn - g, u = H y = H gn - H g = v + p
+  double s[EL];
+  double sTy, yu;
+  {
+    double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+    for (int e = 0; e < EL; ++e) {
+      const double xv = ok[e] ? X[row[e]] : 0.0;
+      const double xtv = ok[e] ? L.xt[row[e] * S::XS + w] : 0.0;
+      const double gv = ok[e] ? G[row[e]] : 0.0;
+      const double pv = ok[e] ? Pd[row[e]] : 0.0;
+      s[e] = xtv - xv;
+      const double y = gn[e] - gv;
+      t0 = ::fma(s[e], y, t0);
+      t1 = ::fma(y, v[e] + pv, t1);
+    }
+    sTy = wave_sum(t0);
+    yu = wave_sum(t1);
+  }
+  const bool upd = !(::fabs(sTy) < kCurvatureSkip);
+  if (upd) {
+    const double rho = 1.0 / sTy;
+    const double beta = ::fma(rho * rho, yu, rho);
+    const double hb = 0.5 * beta;
+    double a[EL];
+    double q0, q1;
+    {
+      double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+      for (int e = 0; e < EL; ++e) {
+        const double pv = ok[e] ? Pd[row[e]] : 0.0;
+        a[e] = ok[e] ? ::fma(hb, s[e], -(rho * (v[e] + pv))) : 0.0;
+        const double ga = acc ? gn[e] : (ok[e] ? G[row[e]] : 0.0);
+        t0 = ::fma(a[e], ga, t0);
+        t1 = ::fma(s[e], ga, t1);
+      }
+      q0 = wave_sum(t0);
+      q1 = wave_sum(t1);
+    }
+    // p' = -H' g = -(H g + s (a.g) + a (s.g)), H g = v (accepted) or -p
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      if (ok[e]) {
+        const double hg = acc ? v[e] : -Pd[row[e]];
+        Pd[row[e]] = -::fma(a[e], q1, ::fma(s[e], q0, hg));
+      }
+    // append the pair; a full list is folded into B first
+    if (st.k == S::kHist) {
+#pragma unroll 1
+      for (int c = 0; c < D; ++c) {
+        double bc[EL];
+#pragma unroll
+        for (int e = 0; e < EL; ++e)
+          bc[e] = st.base ? (ok[e] ? B[int64_t(c) * D + row[e]] : 0.0)
+                          : (row[e] == c ? 1.0 : 0.0);
+#pragma unroll 1
+        for (int j = 0; j < S::kHist; ++j) {
+          const double* sj = hist + int64_t(j) * 2 * D;
+          const double* aj = sj + D;
+          const double sc = sj[c], ac = aj[c];
+#pragma unroll
+          for (int e = 0; e < EL; ++e)
+            if (ok[e])
+              bc[e] = bc[e] + (sj[row[e]] * ac + aj[row[e]] * sc);
+        }
+#pragma unroll
+        for (int e = 0; e < EL; ++e)
+          if (ok[e])
+            B[int64_t(c) * D + row[e]] = bc[e];
+      }
+      st.base = 1;
+      st.k = 0;
+    }
+    double* sk = hist + int64_t(st.k) * 2 * D;
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      if (ok[e]) {
+        sk[row[e]] = s[e];
+        sk[D + row[e]] = a[e];
+      }
+    ++st.k;
+  } else if (acc) {
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      if (ok[e])
+        Pd[row[e]] = -v[e];
+  }
+  if (acc) {
+    st.best = fnew;
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      if (ok[e]) {
+        X[row[e]] = L.xt[row[e] * S::XS + w];
+        G[row[e]] = gn[e];
+      }
+  }
+  const double nrm = gnorm_now();
+  if (!::isfinite(nrm) || !::isfinite(fnew)) {
+    finish(kStatusNonFinite, fnew, nrm);
+    return;
+  }
+  if (nrm < P.tol) {
+    if (lane == 0)
+      count_converged(ctl, P.required);
+    finish(kStatusConverged, st.best, nrm);
+    return;
+  }
+  ++st.iter;
+  begin_iteration();
+}
+
+// One (16-row block, k range) unit of G = Cs Xt: every Cs operand of the
+// unit is loaded before the MFMA chain starts (constant trip count, fully
+// unrolled), so the L2 latency is paid once per unit.
+template <int D, int KB0, int KB1>
+ZEUS_D ZEUS_FI mfma_acc_t
+tile_gemm_unit(const double* Cs, const double* xt, int mb, int li, int lk)
+{
+  using S = TileShape<D>;
+  const int r = 16 * mb + li;
+  double av[KB1 - KB0];
+#pragma unroll
+  for (int kb = KB0; kb < KB1; ++kb) {
+    const int c = 4 * kb + lk; // Cs symmetric: element (r, c) read at [c][r]
+    av[kb - KB0] = (r < D && c < D) ? Cs[int64_t(c) * D + r] : 0.0;
+  }
+  mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kb = KB0; kb < KB1; ++kb)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kb - KB0], xt[(4 * kb + lk) * S::XS + li], acc,
+                                               0, 0, 0);
+  return acc;
+}
+
+template <int D>
+__global__ __launch_bounds__(kTileThreads) void
+bfgs_tile_kernel(TileArgs a)
+{
+  using S = TileShape<D>;
+  __shared__ TileLds<D> L;
+  const int tid = threadIdx.x, lane = tid % kWave;
+  const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const int li = lane & 15, lk = lane >> 4;
+  auto slot_mem = [&](int j) {
+    return a.slot_mem + (int64_t(blockIdx.x) * kTileStarts + j) * S::kSlotDoubles;
+  };
+
+  for (int e = tid; e < S::KP * S::XS; e += kTileThreads)
+    L.xt[e] = 0.0; // padding rows stay zero
+
+  for (int64_t tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
+    const int pi = wave_uniform(a.tile_problem[tile]);
+    if (tid == 0)
+      L.prob = a.probs[pi];
+    __syncthreads();
+    const ProblemDesc& P = L.prob;
+    const double* Cs = P.aux;
+    // slots w, w + kTileWaves, ...: every lane of the wavefront writes the
+    // same values.  A problem's tiles are consecutive from
+    // ProblemDesc::block0 (its PSO workgroups, which hold the same starts).
+    for (int j = w; j < kTileStarts; j += kTileWaves) {
+      TileSlot& st = L.slot[j];
+      st = TileSlot{-1, 0, 0, 0, 0, 0, 0, 0, 0.0, 1.0, 0.0};
+      const int64_t sl = (tile - P.block0) * kTileStarts + j; // local start
+      if (sl < P.n_starts) {
+        st.s = P.start0 + sl;
+        double* X = L.x + j * D;
+        for (int d = lane; d < D; d += kWave) {
+          const double v =
+            P.x0 ? P.x0[sl * D + d]
+                 : rng::uniform_in(P.seed, P.stream, uint64_t(sl), uint64_t(d), P.lower, P.upper);
+          X[d] = v;
+          L.xt[d * S::XS + j] = v;
+        }
+        st.phase = 1;
+      }
+    }
+    Control* ctl = a.ctl + pi;
+    __syncthreads();
+
+    for (;;) {
+      // ---- G = Cs Xt by (row block, k half) units and partial values ----
+      for (int u = w; u < S::U; u += kTileWaves) {
+        const int mb = u >> 1, kh = u & 1;
+        const mfma_acc_t acc = kh ? tile_gemm_unit<D, S::KB1, S::KB>(Cs, L.xt, mb, li, lk)
+                                  : tile_gemm_unit<D, 0, S::KB1>(Cs, L.xt, mb, li, lk);
+        double* gp = L.gt + kh * D * S::XS;
+        double fp = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rr = 16 * mb + 4 * q + lk;
+          if (rr < D) {
+            gp[rr * S::XS + li] = acc[q];
+            fp = ::fma(L.xt[rr * S::XS + li], acc[q], fp);
+          }
+        }
+        fp += __shfl_xor(fp, 16, kWave);
+        fp += __shfl_xor(fp, 32, kWave);
+        if (lk == 0)
+          L.fpart[u * kTileStarts + li] = fp;
+      }
+      __syncthreads();
+      // ---- start logic: wavefront w runs its slots one after the other ----
+      for (int j = w; j < kTileStarts; j += kTileWaves) {
+        if (L.slot[j].phase != 0) {
+          double f = 0.0;
+          for (int u = 0; u < S::U; ++u)
+            f += L.fpart[u * kTileStarts + j];
+          tile_slot_step<D>(L, j, P, ctl, a.out, slot_mem(j), readlane_f64(0.5 * f, 0));
+        }
+        if (lane == 0)
+          L.live[j] = L.slot[j].phase != 0;
+      }
+      __syncthreads();
+      int any = 0;
+#pragma unroll
+      for (int k = 0; k < kTileStarts; ++k)
+        any |= L.live[k];
+      if (!any)
+        break;
+    }
+    __syncthreads(); // LDS reuse by the next tile
+  }
+}
+
+} // namespace zeus::dev

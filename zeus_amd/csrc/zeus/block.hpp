@@ -5,15 +5,60 @@
 
 namespace zeus::dev {
 
-// Sum over the 64 lanes of a wavefront by butterfly exchange; every lane
-// ends with the same bits (each step adds the same two operands).
+// One DPP lane exchange of a double (two 32-bit moves).  Every pattern used
+// here reads an active lane of the same row, so no lane reads out of range.
+template <int kCtrl>
+ZEUS_D ZEUS_FI double
+dpp_f64(double v)
+{
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(b)), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(b >> 32)), kCtrl, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+}
+
+ZEUS_D ZEUS_FI double
+readlane_f64(double v, int lane)
+{
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane(int(uint32_t(b)), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(int(uint32_t(b >> 32)), lane);
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+// A value every lane of the wavefront holds identically, re-read from the
+// first active lane so that the compiler treats it as wave-uniform (scalar
+// registers) instead of one copy per lane.
+template <class T>
+ZEUS_D ZEUS_FI T
+wave_uniform(const T& v)
+{
+  static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+  struct Words {
+    uint32_t w[sizeof(T) / 4];
+  };
+  Words b = __builtin_bit_cast(Words, v);
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 4; ++i)
+    b.w[i] = __builtin_amdgcn_readfirstlane(int(b.w[i]));
+  return __builtin_bit_cast(T, b);
+}
+
+// Sum over the 64 lanes of a wavefront, the same bits in every lane (a
+// wave-uniform value).  Within each 16-lane row: DPP exchanges with lane
+// xor 1, xor 2 (quad_perm), then the half-row and row mirrors; each step
+// adds the same two operands in both lanes, so the row sum is bitwise equal
+// across the row.  The four row sums are then combined in the fixed order
+// (r0 + r1) + (r2 + r3).  All lanes of the wavefront must be active.
 ZEUS_D ZEUS_FI double
 wave_sum(double v)
 {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1)
-    v += __shfl_xor(v, off, kWave);
-  return v;
+  v += dpp_f64<0xB1>(v);  // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E>(v);  // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141>(v); // row_half_mirror
+  v += dpp_f64<0x140>(v); // row_mirror
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) +
+         (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 // Deterministic workgroup sums of K values at a time.  Two LDS slots are

@@ -195,113 +195,109 @@ pso_lane_kernel(PsoArgs a, Prov prov)
   publish_best<256, 256, D>(a, pi, P, v, live ? sl : int64_t(1) << 62);
 }
 
-// Gaussian form: a workgroup owns 16 particles; X_tile (16 x D) times Cs
-// (D x D) on FP64 MFMA (v_mfma_f64_16x16x4f64), f_i = 0.5 x_i . (Cs x_i).
-// Wave w computes the 16 x 16 output tiles n = w, w + 4, ...
+// Gaussian form: a 1024-thread workgroup owns 16 particles.  Their objective
+// values come from G = Cs Xt on FP64 MFMA (v_mfma_f64_16x16x4f64), Xt the
+// 16 positions (D x 16, LDS) and Cs = (C + C^T)/2 streamed from L2 as A
+// operands, split into (16-row block, k half) units spread over the 16
+// wavefronts; f_i = 0.5 x_i . (Cs x_i), the per-unit partial dot products
+// added in a fixed order.  Operand layout: A[i][k] on lane i + 16k, B[k][j] on lane
+// j + 16k, result D[i][j] in register i % 4 of lane j + 16 (i / 4).
 typedef double mfma_acc_t __attribute__((ext_vector_type(4)));
 
+constexpr int kPsoTile = 16;      // particles per workgroup (MFMA N)
+constexpr int kPsoTileThreads = 1024;
+
 template <int D>
-__global__ __launch_bounds__(256) void
+__global__ __launch_bounds__(kPsoTileThreads) void
 pso_gaussian_mfma_kernel(PsoArgs a)
 {
-  constexpr int MT = 16;                 // particles per workgroup
-  constexpr int KP = (D + 3) / 4 * 4;    // K padded to the MFMA depth
-  constexpr int NTILE = (D + 15) / 16;   // 16-wide output tiles
-  constexpr int NP = NTILE * 16;
-  constexpr int KC = 16;                 // rows of Cs staged in LDS per chunk
-  constexpr int TPW = (NTILE + 3) / 4;   // output tiles per wavefront
-  __shared__ double xs[MT][KP];
-  __shared__ double ys[MT][NP + 1];
-  __shared__ double cs[KC][NP + 1];
+  constexpr int MB = (D + 15) / 16, KB = (D + 3) / 4, KP = 4 * KB;
+  constexpr int KB1 = (KB + 1) / 2, U = 2 * MB, XS = kPsoTile + 1;
+  constexpr int NW = kPsoTileThreads / kWave;
+  __shared__ double xs[KP * XS]; // positions, [dim][particle]
+  __shared__ double fpart[U * kPsoTile];
+  __shared__ int improved[kPsoTile];
 
   const int pi = a.block_problem[blockIdx.x];
   const ProblemDesc P = a.probs[pi];
   if (a.iter > P.pso_iter)
     return;
-  const int64_t base = int64_t(blockIdx.x - P.block0) * MT; // local index
-  const int tid = threadIdx.x;
+  const int64_t base = int64_t(blockIdx.x - P.block0) * kPsoTile; // local index
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
 
-  // update / init coordinates and stage the tile
-  for (int e = tid; e < MT * KP; e += 256) {
-    const int r = e / KP, d = e % KP;
-    const int64_t sl = base + r;
-    double xv = 0.0;
-    if (sl < P.n_starts && d < D) {
+  // update / init coordinates (coalesced over d) and stage the tile
+  for (int e = tid; e < KP * XS; e += kPsoTileThreads)
+    xs[e] = 0.0;
+  __syncthreads();
+#pragma unroll
+  for (int e = tid; e < kPsoTile * D; e += kPsoTileThreads) {
+    const int j = e / D, d = e % D;
+    const int64_t sl = base + j;
+    if (sl < P.n_starts) {
       const int64_t s = P.start0 + sl;
       pso_coord(a, P, pi, D, s, d);
-      xv = a.X[s * D + d];
+      xs[d * XS + j] = a.X[s * D + d];
     }
-    xs[r][d] = xv;
   }
   __syncthreads();
 
-  // Y = X_tile Cs: Cs streamed through LDS in KC-row chunks (coalesced
-  // global reads, shared by the four wavefronts); wave w owns the output
-  // tiles w, w + 4, ... (operands: A[i][k] on lane i + 16k, B[k][j] on lane
-  // j + 16k; result D[i][j] in register i / 4 of lane j + 16 (i % 4)).
   const double* Cs = P.aux;
-  const int lane = tid & 63, w = tid >> 6;
-  const int li = lane & 15, lk = lane >> 4;
-  mfma_acc_t acc[TPW];
-#pragma unroll
-  for (int t = 0; t < TPW; ++t)
-    acc[t] = mfma_acc_t{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < KP; k0 += KC) {
-    for (int e = tid; e < KC * NP; e += 256) {
-      const int r = e / NP, c = e % NP, k = k0 + r;
-      cs[r][c] = (k < D && c < D) ? Cs[k * D + c] : 0.0;
+  for (int u = w; u < U; u += NW) {
+    const int mb = u >> 1, kh = u & 1;
+    const int r = 16 * mb + li;
+    const int kb0 = kh ? KB1 : 0, kb1 = kh ? KB : KB1;
+    mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int kb = kb0; kb < kb1; ++kb) {
+      const int c = 4 * kb + lk; // Cs symmetric: element (r, c) read at [c][r]
+      const double av = (r < D && c < D) ? Cs[int64_t(c) * D + r] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xs[c * XS + li], acc, 0, 0, 0);
     }
-    __syncthreads();
+    double fp = 0.0;
 #pragma unroll
-    for (int kk = 0; kk < KC; kk += 4) {
-      if (k0 + kk < KP) {
-        const double av = xs[li][k0 + kk + lk];
-#pragma unroll
-        for (int t = 0; t < TPW; ++t) {
-          const int nt = w + 4 * t;
-          if (nt < NTILE)
-            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, cs[kk + lk][nt * 16 + li], acc[t],
-                                                         0, 0, 0);
-        }
-      }
+    for (int q = 0; q < 4; ++q) {
+      const int rr = 16 * mb + 4 * q + lk;
+      if (rr < D)
+        fp = ::fma(xs[rr * XS + li], acc[q], fp);
     }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int t = 0; t < TPW; ++t) {
-    const int nt = w + 4 * t;
-    if (nt < NTILE) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        ys[4 * r + lk][nt * 16 + li] = acc[t][r];
-    }
+    fp += __shfl_xor(fp, 16, kWave);
+    fp += __shfl_xor(fp, 32, kWave);
+    if (lk == 0)
+      fpart[u * kPsoTile + li] = fp;
   }
   __syncthreads();
 
-  // f_i = 0.5 sum_j x_ij y_ij: 16 lanes per particle, fixed-order combine
-  const int row = tid >> 4, sub = tid & 15;
-  double t = 0.0;
-  for (int j = sub; j < D; j += 16)
-    t = ::fma(xs[row][j], ys[row][j], t);
-#pragma unroll
-  for (int off = 8; off > 0; off >>= 1)
-    t += __shfl_xor(t, off, 16);
-  const int64_t sl = base + row;
-  const bool live = sl < P.n_starts;
-  const int64_t s = P.start0 + sl;
+  // f_i, pBest on strict improvement
   double v = __builtin_nan("");
-  if (live && sub == 0) {
-    const double fv = 0.5 * t;
-    if (a.iter == 0) {
-      a.pbv[s] = fv;
-    } else if (fv < a.pbv[s]) {
-      a.pbv[s] = fv;
-      for (int d = 0; d < D; ++d)
-        a.pbx[s * D + d] = xs[row][d];
+  int64_t vi = int64_t(1) << 62;
+  if (tid < kPsoTile) {
+    const int64_t sl = base + tid;
+    int imp = 0;
+    if (sl < P.n_starts) {
+      double t = 0.0;
+      for (int u = 0; u < U; ++u)
+        t += fpart[u * kPsoTile + tid];
+      const double fv = 0.5 * t;
+      const int64_t s = P.start0 + sl;
+      if (a.iter == 0) {
+        a.pbv[s] = fv;
+      } else if (fv < a.pbv[s]) {
+        a.pbv[s] = fv;
+        imp = 1;
+      }
+      v = a.pbv[s];
+      vi = sl;
     }
-    v = a.pbv[s];
+    improved[tid] = imp;
   }
-  publish_best<256, MT, D>(a, pi, P, v, (live && sub == 0) ? sl : int64_t(1) << 62);
+  __syncthreads();
+  for (int e = tid; e < kPsoTile * D; e += kPsoTileThreads) {
+    const int j = e / D, d = e % D;
+    if (improved[j])
+      a.pbx[(P.start0 + base + j) * D + d] = xs[d * XS + j];
+  }
+  publish_best<kPsoTileThreads, kPsoTile, D>(a, pi, P, v, vi);
 }
 
 } // namespace zeus::dev

@@ -91,7 +91,8 @@ struct DeviceState {
   hipEvent_t ev[6] = {};
   void* arena = nullptr;
   size_t arena_bytes = 0;
-  std::map<TuneKey, TuneVal> tuned;
+  std::map<TuneKey, TuneVal> tuned; // autotuner results (only tuned choices)
+  std::map<std::tuple<int, int, int>, TuneVal> last; // last shape used per class and gradient
 };
 DeviceState g_ds;
 
@@ -235,11 +236,16 @@ std::vector<TuneVal>
 candidates(const ClassOps* ops, const zeus_options& opt)
 {
   std::vector<TuneVal> v;
+  const bool tile_ok = ops->tile_ok && opt.gradient == ZEUS_GRAD_ADJOINT;
+  if (opt.form == ZEUS_FORM_TILE && tile_ok)
+    return {{ZEUS_FORM_TILE, dev::kTileThreads}};
   const bool want_thread = opt.form == ZEUS_FORM_THREAD || opt.form == ZEUS_FORM_AUTO;
   const bool want_group =
     opt.form == ZEUS_FORM_GROUP || opt.form == ZEUS_FORM_AUTO || !ops->thread_ok;
   if (want_thread && ops->thread_ok)
     v.push_back({ZEUS_FORM_THREAD, 0});
+  if (want_group && tile_ok && opt.group_threads <= 0)
+    v.push_back({ZEUS_FORM_TILE, dev::kTileThreads});
   if (want_group || v.empty())
     for (int k = 0; k < ops->n_group; ++k)
       if (opt.group_threads <= 0 || ops->group_nt[k] == opt.group_threads)
@@ -339,10 +345,16 @@ launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
   return hipSuccess;
 }
 
-// Persistent grid of the group form: as many workgroups as can be resident.
+// Persistent grid of the group and tile forms: as many workgroups as can
+// be resident (at most one per start / per tile).
 int
 group_grid(const ClassPlan& c, const TuneVal& v, int gradient)
 {
+  if (v.form == ZEUS_FORM_TILE) {
+    const int occ = c.ops->tile_blocks_per_cu();
+    const int g = int(std::min<int64_t>(c.pso_blocks, int64_t(g_ds.n_cu) * occ));
+    return std::max(g, 1);
+  }
   const int occ = c.ops->group_blocks_per_cu(gradient, v.threads);
   const int g = int(std::min<int64_t>(c.total, int64_t(g_ds.n_cu) * occ));
   return std::max(g, 1);
@@ -353,6 +365,8 @@ slab_doubles(const ClassPlan& c, const TuneVal& v, int gradient)
 {
   if (v.form == ZEUS_FORM_THREAD)
     return c.ops->thread_slab_per_start * c.total;
+  if (v.form == ZEUS_FORM_TILE)
+    return c.ops->tile_slot_doubles * dev::kTileStarts * group_grid(c, v, gradient);
   return c.ops->group_slab_per_block(v.threads) * group_grid(c, v, gradient);
 }
 
@@ -365,6 +379,11 @@ launch_bfgs_cfg(ClassPlan& c, const TuneVal& v, Control* ctl_all, int gradient, 
     return hipSuccess;
   if (v.form == ZEUS_FORM_THREAD)
     return c.ops->bfgs_thread(a, gradient, st);
+  if (v.form == ZEUS_FORM_TILE) {
+    const dev::TileArgs ta{c.d_desc, c.d_block_problem, c.pso_blocks, ctl_all + c.ctl0, c.out,
+                           c.h_slab};
+    return c.ops->bfgs_tile(ta, group_grid(c, v, gradient), st);
+  }
   a.h_slab_stride = c.ops->group_slab_per_block(v.threads);
   return c.ops->bfgs_group(a, gradient, v.threads, group_grid(c, v, gradient), st);
 }
@@ -507,8 +526,6 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
             c.cands = {v};
     }
     c.cfg = c.cands.front();
-    if (c.cands.size() == 1)
-      g_ds.tuned[c.key] = c.cfg;
     c.slab = 0;
     for (const auto& v : c.cands)
       c.slab = std::max(c.slab, slab_doubles(c, v, opt.gradient));
@@ -593,9 +610,11 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     if ((e = launch_class(c, d_ctl, opt.gradient, st)) != hipSuccess)
       return fail(e);
   (void)hipEventRecord(g_ds.ev[1], st);
-  for (auto& c : classes)
+  for (auto& c : classes) {
     if ((e = launch_bfgs(c, d_ctl, opt.gradient, st)) != hipSuccess)
       return fail(e);
+    g_ds.last[{c.ops->objective, c.ops->dim, opt.gradient}] = c.cfg;
+  }
   (void)hipEventRecord(g_ds.ev[2], st);
   for (auto& c : classes) {
     dev::summary_kernel<<<int(c.probs.size()), 256, 0, st>>>(c.d_desc, d_ctl + c.ctl0, c.out,
@@ -979,15 +998,11 @@ zeus_tuned_config(int32_t objective, int32_t dim, int32_t gradient, int32_t* for
   if (!form || !threads)
     return ZEUS_E_ARG;
   std::lock_guard<std::mutex> lock(g_mu);
-  // the largest size bucket tuned for this (objective, dim, gradient)
-  const TuneVal* found = nullptr;
-  for (const auto& kv : g_ds.tuned)
-    if (kv.first.objective == objective && kv.first.dim == dim && kv.first.gradient == gradient)
-      found = &kv.second;
-  if (!found)
+  const auto it = g_ds.last.find({objective, dim, gradient});
+  if (it == g_ds.last.end())
     return ZEUS_E_ARG;
-  *form = found->form;
-  *threads = found->threads;
+  *form = it->second.form;
+  *threads = it->second.threads;
   return ZEUS_OK;
 }
 
