@@ -99,7 +99,9 @@ typedef struct zeus_options {
   int32_t nzerosteps;    /* accepted for API parity; inert (the line search
                             never returns a zero step)                     */
   int32_t use_graph;     /* capture the launch sequence once per batch
-                            shape and replay it as a hipGraph              */
+                            shape and replay it as a hipGraph (once the
+                            autotuner has settled the launch shapes; the
+                            first call of a shape runs uncaptured)         */
   int32_t flags;         /* reserved, 0                                    */
 } zeus_options;
 
@@ -113,8 +115,10 @@ typedef struct zeus_result {
   double fval;
   double gradient_norm;
   int32_t claimed, actual, surrendered, stopped; /* util.hpp:27-32        */
-  double ms_pso;        /* device time of the PSO phase                    */
-  double ms_opt;        /* device time of the BFGS phase                   */
+  double ms_pso;        /* device time of the PSO phase (0 when replayed
+                           from a graph)                                   */
+  double ms_opt;        /* device time of the BFGS phase (of the whole
+                           replayed graph when use_graph)                  */
   double ms_total;      /* host wall time of the call                      */
   int64_t sum_iter;     /* sum over starts of BFGS iterations              */
   int64_t sum_ls_evals; /* sum over starts of line-search evaluations      */

@@ -145,3 +145,20 @@ def test_tile_is_tuned_for_the_headline():
     assert r.status == 1 and r.c.claimed == 512
     form, threads = z.tuned_config(4, 150, 0)
     assert form in (2, 3) and threads in (256, 512, 1024)
+
+
+def test_graph_replays_match_eager():
+    """use_graph: the launch sequence is captured once per batch shape and
+    replayed; every replay (with eager calls in between) gives the eager
+    results bit for bit (the kernels are deterministic)."""
+    C = gauss_matrix(32)
+    probs = [z.Problem(z.Gaussian(C), -5, 5, 40, 10000, 3, 41, 1e-8, 7),
+             z.Problem(z.Rosenbrock(2), -5, 5, 300, 10000, 5, 301, 1e-8, 8)]
+    eager = z.run_batch(probs, per_start=True)
+    for k in range(4):  # tune, capture, replay, replay
+        g = z.run_batch(probs, per_start=True, use_graph=True)
+        for a, b in zip(eager, g):
+            for key in ("status", "iter", "fval", "x"):
+                assert np.array_equal(a.per_start[key], b.per_start[key]), (k, key)
+            assert a.idx == b.idx and a.c.claimed == b.c.claimed
+        z.run_batch(probs[1:], per_start=False)  # eager work between replays

@@ -29,12 +29,12 @@
 
 #include "batch.hpp"
 #include "block.hpp"
+#include "cs_product.hpp"
 #include "philox.hpp"
-#include "pso.hpp"
 
 namespace zeus::dev {
 
-constexpr int kTileStarts = 16; // starts per tile = MFMA N dimension
+constexpr int kTileStarts = kCsTile; // starts per tile = MFMA N dimension
 // Workgroup size of the tile form: 8 wavefronts, each running the logic of
 // two starts (a 256-register budget per lane: the Cs operands of a product
 // unit are all in flight at once without spilling).
@@ -44,14 +44,8 @@ constexpr int kSlotsPerWave = kTileStarts / kTileWaves;
 static_assert(kTileStarts % kTileWaves == 0, "whole slots per wavefront");
 
 template <int D>
-struct TileShape {
-  static constexpr int MB = (D + 15) / 16; // 16-row blocks of Cs
-  static constexpr int KB = (D + 3) / 4;   // 4-deep k steps
-  static constexpr int KP = KB * 4;        // padded depth
+struct TileShape : CsShape<D> {
   static constexpr int EL = (D + kWave - 1) / kWave; // rows per lane in start logic
-  static constexpr int XS = kTileStarts + 1;         // padded LDS row stride
-  static constexpr int KB1 = (KB + 1) / 2;           // k steps of the first half
-  static constexpr int U = 2 * MB;                   // (row block, k half) units
   static constexpr int kHist = 32; // rank-2 pairs kept before folding into B
   static constexpr int64_t kSlotDoubles = int64_t(kHist) * 2 * D + int64_t(D) * D;
 };
@@ -248,8 +242,8 @@ tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
     for (int e = 0; e < EL; ++e)
       v[e] = gn[e];
   }
-#pragma unroll 1
-  for (int j = 0; j < st.k; ++j) {
+#pragma unroll 4
+  for (int j = 0; j < st.k; ++j) { // 4 pairs in flight; v accumulates in order j
     double sj[EL], aj[EL];
     load(hist + int64_t(j) * 2 * D, sj);
     load(hist + int64_t(j) * 2 * D + D, aj);
@@ -370,29 +364,6 @@ tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
   begin_iteration();
 }
 
-// One (16-row block, k range) unit of G = Cs Xt: every Cs operand of the
-// unit is loaded before the MFMA chain starts (constant trip count, fully
-// unrolled), so the L2 latency is paid once per unit.
-template <int D, int KB0, int KB1>
-ZEUS_D ZEUS_FI mfma_acc_t
-tile_gemm_unit(const double* Cs, const double* xt, int mb, int li, int lk)
-{
-  using S = TileShape<D>;
-  const int r = 16 * mb + li;
-  double av[KB1 - KB0];
-#pragma unroll
-  for (int kb = KB0; kb < KB1; ++kb) {
-    const int c = 4 * kb + lk; // Cs symmetric: element (r, c) read at [c][r]
-    av[kb - KB0] = (r < D && c < D) ? Cs[int64_t(c) * D + r] : 0.0;
-  }
-  mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int kb = KB0; kb < KB1; ++kb)
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kb - KB0], xt[(4 * kb + lk) * S::XS + li], acc,
-                                               0, 0, 0);
-  return acc;
-}
-
 template <int D>
 __global__ __launch_bounds__(kTileThreads) void
 bfgs_tile_kernel(TileArgs a)
@@ -443,8 +414,7 @@ bfgs_tile_kernel(TileArgs a)
       // ---- G = Cs Xt by (row block, k half) units and partial values ----
       for (int u = w; u < S::U; u += kTileWaves) {
         const int mb = u >> 1, kh = u & 1;
-        const mfma_acc_t acc = kh ? tile_gemm_unit<D, S::KB1, S::KB>(Cs, L.xt, mb, li, lk)
-                                  : tile_gemm_unit<D, 0, S::KB1>(Cs, L.xt, mb, li, lk);
+        const mfma_acc_t acc = cs_unit<D>(Cs, L.xt, u, li, lk);
         double* gp = L.gt + kh * D * S::XS;
         double fp = 0.0;
 #pragma unroll

@@ -1,0 +1,63 @@
+// cs_product.hpp -- G = Cs Xt for a tile of 16 points on FP64 MFMA, shared
+// by the Gaussian PSO and the tile form of the BFGS.
+//
+// Cs = (C + C^T) / 2 (D x D, global memory, L2-resident); Xt holds 16
+// points as columns in LDS, row-major with a padded stride of 17 doubles
+// (rows D .. KP-1 zero).  v_mfma_f64_16x16x4f64 operand layout: A[i][k] on
+// lane i + 16k, B[k][j] on lane j + 16k, result D[i][j] in register i % 4
+// of lane j + 16 (i / 4).  The product is cut into units (16-row block of
+// Cs, half of the k range); a unit loads all its Cs operands before its
+// MFMA chain starts, so the L2 latency is paid once per unit.
+#pragma once
+
+#include "config.hpp"
+
+namespace zeus::dev {
+
+typedef double mfma_acc_t __attribute__((ext_vector_type(4)));
+
+constexpr int kCsTile = 16; // points per tile = MFMA N dimension
+
+template <int D>
+struct CsShape {
+  static constexpr int MB = (D + 15) / 16;   // 16-row blocks of Cs
+  static constexpr int KB = (D + 3) / 4;     // 4-deep k steps
+  static constexpr int KP = KB * 4;          // padded depth (rows of Xt)
+  static constexpr int KB1 = (KB + 1) / 2;   // k steps of the first half
+  static constexpr int U = 2 * MB;           // (row block, k half) units
+  static constexpr int XS = kCsTile + 1;     // padded LDS row stride of Xt / G
+};
+
+template <int D, int KB0, int KB1>
+ZEUS_D ZEUS_FI mfma_acc_t
+cs_unit_range(const double* Cs, const double* xt, int mb, int li, int lk)
+{
+  using S = CsShape<D>;
+  const int r = 16 * mb + li;
+  double av[KB1 - KB0];
+#pragma unroll
+  for (int kb = KB0; kb < KB1; ++kb) {
+    const int c = 4 * kb + lk; // Cs symmetric: element (r, c) read at [c][r]
+    av[kb - KB0] = (r < D && c < D) ? Cs[int64_t(c) * D + r] : 0.0;
+  }
+  mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kb = KB0; kb < KB1; ++kb)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kb - KB0], xt[(4 * kb + lk) * S::XS + li], acc,
+                                               0, 0, 0);
+  return acc;
+}
+
+// Unit u: rows 16 (u / 2) .. + 15 of Cs, k half u % 2.  Lane (li, lk) of the
+// result holds G[16 (u / 2) + 4 q + lk][li] in component q (partial over
+// the unit's k half).
+template <int D>
+ZEUS_D ZEUS_FI mfma_acc_t
+cs_unit(const double* Cs, const double* xt, int u, int li, int lk)
+{
+  using S = CsShape<D>;
+  return (u & 1) ? cs_unit_range<D, S::KB1, S::KB>(Cs, xt, u >> 1, li, lk)
+                 : cs_unit_range<D, 0, S::KB1>(Cs, xt, u >> 1, li, lk);
+}
+
+} // namespace zeus::dev

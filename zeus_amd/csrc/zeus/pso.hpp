@@ -15,6 +15,7 @@
 
 #include "batch.hpp"
 #include "lane_ops.hpp"
+#include "cs_product.hpp"
 #include "philox.hpp"
 
 namespace zeus::dev {
@@ -196,26 +197,20 @@ pso_lane_kernel(PsoArgs a, Prov prov)
 }
 
 // Gaussian form: a 1024-thread workgroup owns 16 particles.  Their objective
-// values come from G = Cs Xt on FP64 MFMA (v_mfma_f64_16x16x4f64), Xt the
-// 16 positions (D x 16, LDS) and Cs = (C + C^T)/2 streamed from L2 as A
-// operands, split into (16-row block, k half) units spread over the 16
-// wavefronts; f_i = 0.5 x_i . (Cs x_i), the per-unit partial dot products
-// added in a fixed order.  Operand layout: A[i][k] on lane i + 16k, B[k][j] on lane
-// j + 16k, result D[i][j] in register i % 4 of lane j + 16 (i / 4).
-typedef double mfma_acc_t __attribute__((ext_vector_type(4)));
-
-constexpr int kPsoTile = 16;      // particles per workgroup (MFMA N)
+// values come from G = Cs Xt on FP64 MFMA (cs_product.hpp), Xt the 16
+// positions in LDS; f_i = 0.5 x_i . (Cs x_i), the per-unit partial dot
+// products added in a fixed order.
+constexpr int kPsoTile = kCsTile; // particles per workgroup (MFMA N)
 constexpr int kPsoTileThreads = 1024;
 
 template <int D>
 __global__ __launch_bounds__(kPsoTileThreads) void
 pso_gaussian_mfma_kernel(PsoArgs a)
 {
-  constexpr int MB = (D + 15) / 16, KB = (D + 3) / 4, KP = 4 * KB;
-  constexpr int KB1 = (KB + 1) / 2, U = 2 * MB, XS = kPsoTile + 1;
+  using S = CsShape<D>;
   constexpr int NW = kPsoTileThreads / kWave;
-  __shared__ double xs[KP * XS]; // positions, [dim][particle]
-  __shared__ double fpart[U * kPsoTile];
+  __shared__ double xs[S::KP * S::XS]; // positions, [dim][particle]
+  __shared__ double fpart[S::U * kPsoTile];
   __shared__ int improved[kPsoTile];
 
   const int pi = a.block_problem[blockIdx.x];
@@ -227,7 +222,7 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   const int li = lane & 15, lk = lane >> 4;
 
   // update / init coordinates (coalesced over d) and stage the tile
-  for (int e = tid; e < KP * XS; e += kPsoTileThreads)
+  for (int e = tid; e < S::KP * S::XS; e += kPsoTileThreads)
     xs[e] = 0.0;
   __syncthreads();
 #pragma unroll
@@ -237,29 +232,20 @@ pso_gaussian_mfma_kernel(PsoArgs a)
     if (sl < P.n_starts) {
       const int64_t s = P.start0 + sl;
       pso_coord(a, P, pi, D, s, d);
-      xs[d * XS + j] = a.X[s * D + d];
+      xs[d * S::XS + j] = a.X[s * D + d];
     }
   }
   __syncthreads();
 
   const double* Cs = P.aux;
-  for (int u = w; u < U; u += NW) {
-    const int mb = u >> 1, kh = u & 1;
-    const int r = 16 * mb + li;
-    const int kb0 = kh ? KB1 : 0, kb1 = kh ? KB : KB1;
-    mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
-    for (int kb = kb0; kb < kb1; ++kb) {
-      const int c = 4 * kb + lk; // Cs symmetric: element (r, c) read at [c][r]
-      const double av = (r < D && c < D) ? Cs[int64_t(c) * D + r] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xs[c * XS + li], acc, 0, 0, 0);
-    }
+  for (int u = w; u < S::U; u += NW) {
+    const mfma_acc_t acc = cs_unit<D>(Cs, xs, u, li, lk);
     double fp = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int rr = 16 * mb + 4 * q + lk;
+      const int rr = 16 * (u >> 1) + 4 * q + lk;
       if (rr < D)
-        fp = ::fma(xs[rr * XS + li], acc[q], fp);
+        fp = ::fma(xs[rr * S::XS + li], acc[q], fp);
     }
     fp += __shfl_xor(fp, 16, kWave);
     fp += __shfl_xor(fp, 32, kWave);
@@ -276,7 +262,7 @@ pso_gaussian_mfma_kernel(PsoArgs a)
     int imp = 0;
     if (sl < P.n_starts) {
       double t = 0.0;
-      for (int u = 0; u < U; ++u)
+      for (int u = 0; u < S::U; ++u)
         t += fpart[u * kPsoTile + tid];
       const double fv = 0.5 * t;
       const int64_t s = P.start0 + sl;
@@ -295,7 +281,7 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   for (int e = tid; e < kPsoTile * D; e += kPsoTileThreads) {
     const int j = e / D, d = e % D;
     if (improved[j])
-      a.pbx[(P.start0 + base + j) * D + d] = xs[d * XS + j];
+      a.pbx[(P.start0 + base + j) * D + d] = xs[d * S::XS + j];
   }
   publish_best<kPsoTileThreads, kPsoTile, D>(a, pi, P, v, vi);
 }
