@@ -31,6 +31,7 @@ struct ProblemDesc {
   int64_t n_params;
   const double* x0;     // optional device start points (n_starts x dim)
   const double* aux;    // library-prepared objective data (Gaussian: Cs)
+  const double* aux_frag; // Gaussian: Cs in MFMA A-operand order (cs_product.hpp)
 };
 
 // Per-problem control block; zeroed before each call.

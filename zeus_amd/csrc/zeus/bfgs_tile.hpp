@@ -35,10 +35,9 @@
 namespace zeus::dev {
 
 constexpr int kTileStarts = kCsTile; // starts per tile = MFMA N dimension
-// Workgroup size of the tile form: 8 wavefronts, each running the logic of
-// two starts (a 256-register budget per lane: the Cs operands of a product
-// unit are all in flight at once without spilling).
-constexpr int kTileThreads = 512;
+// Workgroup size of the tile form: 16 wavefronts, one per start of the
+// tile (a 128-register budget per lane).
+constexpr int kTileThreads = 1024;
 constexpr int kTileWaves = kTileThreads / kWave;
 constexpr int kSlotsPerWave = kTileStarts / kTileWaves;
 static_assert(kTileStarts % kTileWaves == 0, "whole slots per wavefront");
@@ -76,6 +75,7 @@ struct TileLds {
   double fpart[S::U * kTileStarts];
   double x[kTileStarts * D], g[kTileStarts * D], p[kTileStarts * D];
   int live[kTileStarts];
+  int stop; // early stop seen at the start of this round
   TileSlot slot[kTileStarts];
   ProblemDesc prob; // the tile's problem
 };
@@ -175,7 +175,7 @@ tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
       finish(kStatusSurrendered, st.best, gnorm_now());
       return;
     }
-    if (load_flag(&ctl->stop_flag) != 0) {
+    if (L.stop != 0) {
       finish(kStatusStopped, st.best, gnorm_now());
       return;
     }
@@ -242,8 +242,8 @@ tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
     for (int e = 0; e < EL; ++e)
       v[e] = gn[e];
   }
-#pragma unroll 4
-  for (int j = 0; j < st.k; ++j) { // 4 pairs in flight; v accumulates in order j
+#pragma unroll 2
+  for (int j = 0; j < st.k; ++j) { // 2 pairs in flight; v accumulates in order j
     double sj[EL], aj[EL];
     load(hist + int64_t(j) * 2 * D, sj);
     load(hist + int64_t(j) * 2 * D + D, aj);
@@ -356,7 +356,8 @@ tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
   }
   if (nrm < P.tol) {
     if (lane == 0)
-      count_converged(ctl, P.required);
+      (void)__hip_atomic_fetch_add(&ctl->converged, 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT); // no wait for the old value
     finish(kStatusConverged, st.best, nrm);
     return;
   }
@@ -386,7 +387,7 @@ bfgs_tile_kernel(TileArgs a)
       L.prob = a.probs[pi];
     __syncthreads();
     const ProblemDesc& P = L.prob;
-    const double* Cs = P.aux;
+    const double* Cf = P.aux_frag;
     // slots w, w + kTileWaves, ...: every lane of the wavefront writes the
     // same values.  A problem's tiles are consecutive from
     // ProblemDesc::block0 (its PSO workgroups, which hold the same starts).
@@ -411,10 +412,19 @@ bfgs_tile_kernel(TileArgs a)
     __syncthreads();
 
     for (;;) {
+      // Early stop (reference bfgs_common.cuh:170-208): the problem stops
+      // once `required` starts have converged.  Read once per round, before
+      // the product, so its latency hides behind the MFMA work; a start sees
+      // it at the top of its next iteration, as in the other forms.
+      if (tid == 0) {
+        const int conv = __hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+        L.stop = P.required > 0 && conv >= P.required;
+      }
       // ---- G = Cs Xt by (row block, k half) units and partial values ----
       for (int u = w; u < S::U; u += kTileWaves) {
         const int mb = u >> 1, kh = u & 1;
-        const mfma_acc_t acc = cs_unit<D>(Cs, L.xt, u, li, lk);
+        const mfma_acc_t acc = cs_unit<D>(Cf, L.xt, u, li, lk);
         double* gp = L.gt + kh * D * S::XS;
         double fp = 0.0;
 #pragma unroll
@@ -434,10 +444,11 @@ bfgs_tile_kernel(TileArgs a)
       // ---- start logic: wavefront w runs its slots one after the other ----
       for (int j = w; j < kTileStarts; j += kTileWaves) {
         if (L.slot[j].phase != 0) {
-          double f = 0.0;
-          for (int u = 0; u < S::U; ++u)
-            f += L.fpart[u * kTileStarts + j];
-          tile_slot_step<D>(L, j, P, ctl, a.out, slot_mem(j), readlane_f64(0.5 * f, 0));
+          // f = 0.5 sum_u fpart[u]: lane u holds unit u's partial, one
+          // fixed-order wavefront sum
+          static_assert(S::U <= kWave, "one unit per lane");
+          const double fu = lane < S::U ? L.fpart[lane * kTileStarts + j] : 0.0;
+          tile_slot_step<D>(L, j, P, ctl, a.out, slot_mem(j), 0.5 * wave_sum(fu));
         }
         if (lane == 0)
           L.live[j] = L.slot[j].phase != 0;

@@ -1,13 +1,16 @@
 // cs_product.hpp -- G = Cs Xt for a tile of 16 points on FP64 MFMA, shared
 // by the Gaussian PSO and the tile form of the BFGS.
 //
-// Cs = (C + C^T) / 2 (D x D, global memory, L2-resident); Xt holds 16
-// points as columns in LDS, row-major with a padded stride of 17 doubles
-// (rows D .. KP-1 zero).  v_mfma_f64_16x16x4f64 operand layout: A[i][k] on
-// lane i + 16k, B[k][j] on lane j + 16k, result D[i][j] in register i % 4
-// of lane j + 16 (i / 4).  The product is cut into units (16-row block of
-// Cs, half of the k range); a unit loads all its Cs operands before its
-// MFMA chain starts, so the L2 latency is paid once per unit.
+// Cs = (C + C^T) / 2 in MFMA A-operand order (fragment (mb, kb) = 64
+// consecutive doubles, lane l holding Cs[16 mb + l % 16][4 kb + l / 16], zero
+// outside D x D; prepared once per call, L2-resident); Xt holds 16 points
+// as columns in LDS, row-major with a padded stride of 17 doubles (rows D ..
+// KP-1 zero).  v_mfma_f64_16x16x4f64 operand layout: A[i][k] on lane
+// i + 16k, B[k][j] on lane j + 16k, result D[i][j] in register i % 4 of lane
+// j + 16 (i / 4).  The product is cut into units (16-row block of Cs, half
+// of the k range); a unit loads all its Cs fragments (one coalesced 512-byte
+// read each) before its MFMA chain starts, so the L2 latency is paid once
+// per unit.
 #pragma once
 
 #include "config.hpp"
@@ -30,16 +33,14 @@ struct CsShape {
 
 template <int D, int KB0, int KB1>
 ZEUS_D ZEUS_FI mfma_acc_t
-cs_unit_range(const double* Cs, const double* xt, int mb, int li, int lk)
+cs_unit_range(const double* Cf, const double* xt, int mb, int li, int lk)
 {
   using S = CsShape<D>;
-  const int r = 16 * mb + li;
+  const double* frag = Cf + (int64_t(mb) * S::KB << 6) + (li + 16 * lk);
   double av[KB1 - KB0];
 #pragma unroll
-  for (int kb = KB0; kb < KB1; ++kb) {
-    const int c = 4 * kb + lk; // Cs symmetric: element (r, c) read at [c][r]
-    av[kb - KB0] = (r < D && c < D) ? Cs[int64_t(c) * D + r] : 0.0;
-  }
+  for (int kb = KB0; kb < KB1; ++kb)
+    av[kb - KB0] = frag[kb << 6];
   mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int kb = KB0; kb < KB1; ++kb)
@@ -53,11 +54,11 @@ cs_unit_range(const double* Cs, const double* xt, int mb, int li, int lk)
 // the unit's k half).
 template <int D>
 ZEUS_D ZEUS_FI mfma_acc_t
-cs_unit(const double* Cs, const double* xt, int u, int li, int lk)
+cs_unit(const double* Cf, const double* xt, int u, int li, int lk)
 {
   using S = CsShape<D>;
-  return (u & 1) ? cs_unit_range<D, S::KB1, S::KB>(Cs, xt, u >> 1, li, lk)
-                 : cs_unit_range<D, 0, S::KB1>(Cs, xt, u >> 1, li, lk);
+  return (u & 1) ? cs_unit_range<D, S::KB1, S::KB>(Cf, xt, u >> 1, li, lk)
+                 : cs_unit_range<D, 0, S::KB1>(Cf, xt, u >> 1, li, lk);
 }
 
 } // namespace zeus::dev

@@ -237,9 +237,9 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   }
   __syncthreads();
 
-  const double* Cs = P.aux;
+  const double* Cf = P.aux_frag;
   for (int u = w; u < S::U; u += NW) {
-    const mfma_acc_t acc = cs_unit<D>(Cs, xs, u, li, lk);
+    const mfma_acc_t acc = cs_unit<D>(Cf, xs, u, li, lk);
     double fp = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {

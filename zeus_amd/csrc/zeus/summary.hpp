@@ -25,15 +25,33 @@ init_controls_kernel(Control* c, int n, double* gbx, int64_t n_gbx)
     gbx[k] = 0.0;
 }
 
-// Cs = (C + C^T) / 2 for a Gaussian problem (same quadratic form).
+// Cs = (C + C^T) / 2 for a Gaussian problem (same quadratic form), row
+// major, and the same matrix in MFMA A-operand order: fragment (mb, kb) is
+// 64 consecutive doubles, lane l holding Cs[16 mb + l % 16][4 kb + l / 16]
+// (zero outside D x D), so a fragment load is one coalesced 512-byte read.
 static __global__ void
-symmetrize_kernel(const double* C, double* Cs, int D)
+symmetrize_kernel(const double* C, double* Cs, double* Cf, int D)
 {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e < int64_t(D) * D) {
     const int i = int(e / D), j = int(e % D);
     Cs[e] = 0.5 * (C[int64_t(i) * D + j] + C[int64_t(j) * D + i]);
   }
+  const int KB = (D + 3) / 4, MB = (D + 15) / 16;
+  if (e < int64_t(MB) * KB * 64) {
+    const int lane = int(e & 63);
+    const int64_t f = e >> 6;
+    const int mb = int(f / KB), kb = int(f % KB);
+    const int r = 16 * mb + (lane & 15), c = 4 * kb + (lane >> 4);
+    Cf[e] = (r < D && c < D) ? 0.5 * (C[int64_t(r) * D + c] + C[int64_t(c) * D + r]) : 0.0;
+  }
+}
+
+// Doubles of the Gaussian preparation per problem: Cs, then its fragments.
+ZEUS_HD constexpr int64_t
+gaussian_aux_doubles(int D)
+{
+  return int64_t(D) * D + int64_t((D + 15) / 16) * ((D + 3) / 4) * 64;
 }
 
 // One workgroup per problem: deterministic argmin of the start values (the

@@ -321,7 +321,8 @@ carve_class(Carver& cv, ClassPlan& c)
   c.out.fval = cv.take<double>(n);
   c.out.gnorm = cv.take<double>(n);
   c.out.x = cv.take<double>(n * D);
-  c.aux = cv.take<double>(c.ops->objective == obj::kGaussian ? np * D * D : 1);
+  c.aux = cv.take<double>(c.ops->objective == obj::kGaussian ? np * dev::gaussian_aux_doubles(D)
+                                                              : 1);
   c.h_slab = cv.take<double>(size_t(std::max<int64_t>(c.slab, 1)));
 }
 
@@ -333,10 +334,11 @@ launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
   Control* ctl = ctl_all + c.ctl0;
   hipError_t e = hipSuccess;
   if (c.ops->objective == obj::kGaussian) {
+    const int64_t na = dev::gaussian_aux_doubles(D);
     for (int k = 0; k < np; ++k) {
-      const int64_t nn = int64_t(D) * D;
-      dev::symmetrize_kernel<<<int((nn + 255) / 256), 256, 0, st>>>(c.desc[k].params,
-                                                                    c.aux + k * nn, D);
+      double* cs = c.aux + k * na;
+      dev::symmetrize_kernel<<<int((na + 255) / 256), 256, 0, st>>>(c.desc[k].params, cs,
+                                                                    cs + int64_t(D) * D, D);
     }
     if ((e = hipGetLastError()) != hipSuccess)
       return e;
@@ -566,8 +568,10 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     for (auto& d : c.desc) {
       if (d.x0 == nullptr && d.pso_iter >= 0)
         d.x0 = c.pbx + d.start0 * d.dim;
-      if (c.ops->objective == obj::kGaussian)
-        d.aux = c.aux + (&d - c.desc.data()) * int64_t(d.dim) * d.dim;
+      if (c.ops->objective == obj::kGaussian) {
+        d.aux = c.aux + (&d - c.desc.data()) * dev::gaussian_aux_doubles(d.dim);
+        d.aux_frag = d.aux + int64_t(d.dim) * d.dim;
+      }
     }
   // best-x offsets per control block (class order)
   std::vector<int64_t> bxo(ctl_total);
@@ -863,8 +867,10 @@ zeus_pso_run(const zeus_problem* prob, double* pbest_x, double* pbest_val, doubl
   c.pbv = pbest_val;
   if (gbest_x)
     c.gbx = gbest_x;
-  if (c.ops->objective == obj::kGaussian)
+  if (c.ops->objective == obj::kGaussian) {
     c.desc[0].aux = c.aux;
+    c.desc[0].aux_frag = c.aux + int64_t(prob->dim) * prob->dim;
+  }
   hipError_t e = hipMemcpyAsync(c.d_desc, c.desc.data(), sizeof(ProblemDesc),
                                 hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
