@@ -37,6 +37,18 @@ struct Ops {
     return hipGetLastError();
   }
 
+  // gBest after each problem's last iteration (one workgroup per problem)
+  static hipError_t pso_final(const dev::PsoArgs& a, int n_probs, hipStream_t s)
+  {
+    if (fault_take_launch())
+      return hipErrorLaunchFailure;
+    if constexpr (kMfmaPso)
+      dev::pso_final_kernel<256, dev::kPsoTile, D><<<n_probs, 256, 0, s>>>(a);
+    else
+      dev::pso_final_kernel<256, 256, D><<<n_probs, 256, 0, s>>>(a);
+    return hipGetLastError();
+  }
+
   static hipError_t bfgs_thread(const dev::ClassArgs& a, int gradient, hipStream_t s)
   {
     if constexpr (!kThreadOk) {
@@ -194,6 +206,7 @@ struct Ops {
       o.dim = D;
       o.pso_ppb = kMfmaPso ? dev::kPsoTile : 256;
       o.pso = &pso;
+      o.pso_final = &pso_final;
       o.thread_ok = kThreadOk;
       o.bfgs_thread = &bfgs_thread;
       o.thread_slab_per_start = D <= dev::kThreadRegMaxD ? 0 : int64_t(D) * (D + 1) / 2;

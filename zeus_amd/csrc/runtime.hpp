@@ -18,6 +18,7 @@ struct ClassOps {
   int dim = 0;
   int pso_ppb = 256; // particles per PSO workgroup
   hipError_t (*pso)(const dev::PsoArgs&, int nblocks, hipStream_t) = nullptr;
+  hipError_t (*pso_final)(const dev::PsoArgs&, int n_probs, hipStream_t) = nullptr;
   bool thread_ok = false;
   hipError_t (*bfgs_thread)(const dev::ClassArgs&, int gradient, hipStream_t) = nullptr;
   int64_t thread_slab_per_start = 0; // doubles of global H per start

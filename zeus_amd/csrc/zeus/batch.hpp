@@ -38,7 +38,7 @@ struct ProblemDesc {
 struct Control {
   int32_t stop_flag;       // set once `required` starts have converged
   int32_t converged;       // number of status-1 starts so far
-  uint32_t pso_arrive;     // PSO "last workgroup" counter
+  uint32_t pad0;
   int32_t best_idx;        // argmin start (local index), summary kernel
   double gbest_val;        // PSO global best value
   int32_t claimed, stopped, surrendered, nonfinite, actual, pad1;

@@ -413,14 +413,14 @@ bfgs_tile_kernel(TileArgs a)
 
     for (;;) {
       // Early stop (reference bfgs_common.cuh:170-208): the problem stops
-      // once `required` starts have converged.  Read once per round, before
-      // the product, so its latency hides behind the MFMA work; a start sees
-      // it at the top of its next iteration, as in the other forms.
-      if (tid == 0) {
-        const int conv = __hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-        L.stop = P.required > 0 && conv >= P.required;
-      }
+      // once `required` starts have converged.  Read once per round by the
+      // last wavefront (one product unit) before its unit and used after it,
+      // so the load's latency hides behind the MFMA work; a start sees it at
+      // the top of its next iteration, as in the other forms.
+      constexpr int kStopReader = kTileThreads - kWave;
+      int conv = 0;
+      if (tid == kStopReader)
+        conv = __hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       // ---- G = Cs Xt by (row block, k half) units and partial values ----
       for (int u = w; u < S::U; u += kTileWaves) {
         const int mb = u >> 1, kh = u & 1;
@@ -440,6 +440,8 @@ bfgs_tile_kernel(TileArgs a)
         if (lk == 0)
           L.fpart[u * kTileStarts + li] = fp;
       }
+      if (tid == kStopReader)
+        L.stop = P.required > 0 && conv >= P.required;
       __syncthreads();
       // ---- start logic: wavefront w runs its slots one after the other ----
       for (int j = w; j < kTileStarts; j += kTileWaves) {

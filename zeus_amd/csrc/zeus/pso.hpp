@@ -6,16 +6,24 @@
 // x' = x + v' (no clamping), pBest on strict improvement.  The result is
 // pBestX.  Designed-out hazards of the reference: gBest is frozen during an
 // iteration and replaced after it -- only on strict improvement, ties to the
-// lowest particle index -- by the last workgroup of the problem to finish
-// (wavefront-shuffle argmin, then LDS across wavefronts, then the
-// per-workgroup partials), so runs are reproducible and need no host sync.
+// lowest particle index -- so runs are reproducible and need no host sync.
+//
+// One launch per iteration.  A launch ends by publishing, per workgroup, its
+// best particle (value, index, coordinates) in partial slots kept by
+// iteration parity.  The next launch starts with every workgroup of a
+// problem folding that problem's partials itself, in one total order, into
+// the gBest the iteration uses; the problem's first workgroup also stores
+// it, for the fold after.  Visibility between launches is the kernel
+// boundary: no device-scope atomics or fences inside the kernels (on a
+// multi-die GPU those write back and invalidate the L2 of every die, at
+// tens of microseconds each).
 //
 // Buffers are start-major: X[s * D + d] for global start s of the class.
 #pragma once
 
 #include "batch.hpp"
-#include "lane_ops.hpp"
 #include "cs_product.hpp"
+#include "lane_ops.hpp"
 #include "philox.hpp"
 
 namespace zeus::dev {
@@ -24,14 +32,17 @@ struct PsoArgs {
   const ProblemDesc* probs;
   int32_t n_probs;
   int32_t iter;        // 0 = init, t >= 1 = iteration t
+  int32_t n_blocks;    // PSO workgroups of the class (partial slots per parity)
   Control* ctl;
   double* X;
   double* V;
   double* pbx;
   double* pbv;
-  double* gbx;         // n_probs x D
-  double* part_val;    // one per workgroup
-  int64_t* part_idx;
+  double* gbx;         // n_probs x D: gBest after the last iteration (final fold)
+  double* part_val;    // [2][n_blocks]: best pBest value of each workgroup
+  int64_t* part_idx;   // [2][n_blocks]: its problem-local particle index
+  double* part_x;      // [2][n_blocks][D]: its coordinates
+  double* gstate;      // n_probs x [2][1 + D]: gBest (value, x) by iteration parity
   const int32_t* block_problem; // workgroup -> problem
 };
 
@@ -83,64 +94,64 @@ block_argmin(double& v, int64_t& i, double* sv, int64_t* si)
   }
 }
 
-// After the workgroup's particles are final: publish the workgroup's best,
-// and let the last workgroup of the problem fold the partials into gBest.
+// gBest after iteration t_prev, computed by every workgroup of the problem
+// from the same inputs: the gBest before it (stored during iteration t_prev;
+// +inf before iteration 0) and the partials iteration t_prev published.
+// Replaced only on strict improvement.  Writes gb[0..D) (LDS) and returns
+// the value; with `store` (the problem's first PSO workgroup) the result is
+// kept for the next fold.  All threads of the workgroup call.
 template <int NT, int PPB, int D>
-ZEUS_D void
-publish_best(const PsoArgs& a, int pi, const ProblemDesc& P, double v, int64_t idx)
+ZEUS_D double
+pso_fold(const PsoArgs& a, int pi, const ProblemDesc& P, int t_prev, double* gb, bool store)
 {
   __shared__ double sv[NT / kWave];
   __shared__ int64_t si[NT / kWave];
-  __shared__ int last;
-  block_argmin<NT>(v, idx, sv, si);
-  const int nblk = int((P.n_starts + PPB - 1) / PPB); // PPB particles per workgroup
-  if (threadIdx.x == 0) {
-    a.part_val[blockIdx.x] = v;
-    a.part_idx[blockIdx.x] = idx;
-    __threadfence();
-    const uint32_t t = __hip_atomic_fetch_add(&a.ctl[pi].pso_arrive, 1u, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == uint32_t(nblk - 1));
-  }
-  __syncthreads();
-  if (!last)
-    return;
-  __threadfence();
-  // fold the partials with the whole workgroup; `better` is a total order,
-  // so the result does not depend on the folding order
+  __shared__ double s_val;
+  __shared__ int s_src; // -1: keep the previous gBest, else winning workgroup
+  const int q = t_prev & 1;
+  const int nblk = int((P.n_starts + PPB - 1) / PPB);
   double bv = __builtin_nan("");
   int64_t bi = int64_t(1) << 62;
   for (int b = threadIdx.x; b < nblk; b += NT) {
-    const double ov = a.part_val[P.block0 + b];
-    const int64_t oi = a.part_idx[P.block0 + b];
-    if (better(ov, oi, bv, bi)) {
-      bv = ov;
-      bi = oi;
+    const int64_t o = int64_t(q) * a.n_blocks + P.block0 + b;
+    if (better(a.part_val[o], a.part_idx[o], bv, bi)) {
+      bv = a.part_val[o];
+      bi = a.part_idx[o];
     }
   }
-  __syncthreads(); // sv/si are reused by the second argmin
   block_argmin<NT>(bv, bi, sv, si);
+  double* gs = a.gstate + int64_t(pi) * 2 * (D + 1);
+  const double* gprev = gs + (q ^ 1) * (D + 1); // stored during iteration t_prev
   if (threadIdx.x == 0) {
-    a.ctl[pi].pso_arrive = 0u;
-    if (bv < a.ctl[pi].gbest_val) {
-      a.ctl[pi].gbest_val = bv;
-      sv[0] = 1.0;
-      si[0] = bi;
-    } else {
-      sv[0] = 0.0;
+    const double pv = t_prev == 0 ? __builtin_huge_val() : gprev[0];
+    s_src = -1;
+    s_val = pv;
+    if (bv < pv) {
+      s_val = bv;
+      s_src = int(bi / PPB);
     }
   }
   __syncthreads();
-  if (sv[0] != 0.0) {
-    const int64_t src = P.start0 + si[0]; // winner's local index -> global row
+  const double* src = s_src >= 0 ? a.part_x + (int64_t(q) * a.n_blocks + P.block0 + s_src) * D
+                                 : gprev + 1;
+  for (int d = threadIdx.x; d < D; d += NT)
+    gb[d] = src[d];
+  const double val = s_val;
+  if (store) {
+    double* gnew = gs + q * (D + 1);
+    if (threadIdx.x == 0)
+      gnew[0] = val;
     for (int d = threadIdx.x; d < D; d += NT)
-      a.gbx[int64_t(pi) * D + d] = a.pbx[src * D + d];
+      gnew[1 + d] = gb[d];
   }
+  __syncthreads();
+  return val;
 }
 
 // Position/velocity of one coordinate: init (t = 0) or update (t >= 1).
-ZEUS_D ZEUS_FI void
-pso_coord(const PsoArgs& a, const ProblemDesc& P, int pi, int D, int64_t s, int d)
+// Returns the new position.
+ZEUS_D ZEUS_FI double
+pso_coord(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d, const double* gb)
 {
   const int64_t sl = s - P.start0;
   const int64_t o = s * D + d;
@@ -153,17 +164,27 @@ pso_coord(const PsoArgs& a, const ProblemDesc& P, int pi, int D, int64_t s, int 
     a.X[o] = rx;
     a.V[o] = rv;
     a.pbx[o] = rx;
-  } else {
-    const double r1 = rng::uniform_in(P.seed, P.stream, uint64_t(sl),
-                                      rng::pso_draw(D, a.iter, d, 0), 0.0, 1.0);
-    const double r2 = rng::uniform_in(P.seed, P.stream, uint64_t(sl),
-                                      rng::pso_draw(D, a.iter, d, 1), 0.0, 1.0);
-    const double xi = a.X[o], vi = a.V[o], pb = a.pbx[o];
-    const double gb = a.gbx[int64_t(pi) * D + d];
-    const double nv = kPsoW * vi + kPsoC1 * r1 * (pb - xi) + kPsoC2 * r2 * (gb - xi);
-    a.V[o] = nv;
-    a.X[o] = xi + nv;
+    return rx;
   }
+  const double r1 = rng::uniform_in(P.seed, P.stream, uint64_t(sl),
+                                    rng::pso_draw(D, a.iter, d, 0), 0.0, 1.0);
+  const double r2 = rng::uniform_in(P.seed, P.stream, uint64_t(sl),
+                                    rng::pso_draw(D, a.iter, d, 1), 0.0, 1.0);
+  const double xi = a.X[o], vi = a.V[o], pb = a.pbx[o];
+  const double nv = kPsoW * vi + kPsoC1 * r1 * (pb - xi) + kPsoC2 * r2 * (gb[d] - xi);
+  a.V[o] = nv;
+  a.X[o] = xi + nv;
+  return xi + nv;
+}
+
+// Publish the workgroup's best particle into this iteration's partial slot
+// (value and index; the caller copies the coordinates).
+ZEUS_D ZEUS_FI void
+pso_publish_best(const PsoArgs& a, double v, int64_t idx)
+{
+  const int64_t o = int64_t(a.iter & 1) * a.n_blocks + blockIdx.x;
+  a.part_val[o] = v;
+  a.part_idx[o] = idx;
 }
 
 // Generic form: one lane per particle, objective evaluated by that lane.
@@ -171,10 +192,16 @@ template <class F, int D, class Prov>
 __global__ __launch_bounds__(256) void
 pso_lane_kernel(PsoArgs a, Prov prov)
 {
+  __shared__ double gb[D];
+  __shared__ double sv[4];
+  __shared__ int64_t si[4];
+  __shared__ int64_t s_best;
   const int pi = a.block_problem[blockIdx.x];
   const ProblemDesc P = a.probs[pi];
   if (a.iter > P.pso_iter)
-    return; // this problem runs fewer iterations (whole problem returns)
+    return; // this problem runs fewer iterations (whole workgroup returns)
+  if (a.iter > 0)
+    (void)pso_fold<256, 256, D>(a, pi, P, a.iter - 1, gb, int(blockIdx.x) == P.block0);
   const int64_t sl = int64_t(blockIdx.x - P.block0) * 256 + threadIdx.x;
   const bool live = sl < P.n_starts;
   const int64_t s = P.start0 + sl;
@@ -182,7 +209,7 @@ pso_lane_kernel(PsoArgs a, Prov prov)
   if (live) {
     decltype(auto) f = prov.template get<F>(P);
     for (int d = 0; d < D; ++d)
-      pso_coord(a, P, pi, D, s, d);
+      (void)pso_coord(a, P, D, s, d, gb);
     const double fv = lane_value<F, D>(f, a.X + s * D);
     if (a.iter == 0) {
       a.pbv[s] = fv;
@@ -193,7 +220,18 @@ pso_lane_kernel(PsoArgs a, Prov prov)
     }
     v = a.pbv[s];
   }
-  publish_best<256, 256, D>(a, pi, P, v, live ? sl : int64_t(1) << 62);
+  int64_t idx = live ? sl : int64_t(1) << 62;
+  block_argmin<256>(v, idx, sv, si);
+  if (threadIdx.x == 0) {
+    pso_publish_best(a, v, idx);
+    s_best = idx;
+  }
+  __syncthreads();
+  // the best particle's coordinates: its pBest row, final for this launch
+  const int64_t bs = P.start0 + s_best;
+  double* px = a.part_x + (int64_t(a.iter & 1) * a.n_blocks + blockIdx.x) * D;
+  for (int d = threadIdx.x; d < D; d += 256)
+    px[d] = a.pbx[bs * D + d];
 }
 
 // Gaussian form: a 1024-thread workgroup owns 16 particles.  Their objective
@@ -211,7 +249,10 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   constexpr int NW = kPsoTileThreads / kWave;
   __shared__ double xs[S::KP * S::XS]; // positions, [dim][particle]
   __shared__ double fpart[S::U * kPsoTile];
+  __shared__ double gb[D];
+  __shared__ double pbv[kPsoTile];
   __shared__ int improved[kPsoTile];
+  __shared__ int best_j;
 
   const int pi = a.block_problem[blockIdx.x];
   const ProblemDesc P = a.probs[pi];
@@ -221,25 +262,24 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
 
-  // update / init coordinates (coalesced over d) and stage the tile
   for (int e = tid; e < S::KP * S::XS; e += kPsoTileThreads)
     xs[e] = 0.0;
+  if (a.iter > 0)
+    (void)pso_fold<kPsoTileThreads, kPsoTile, D>(a, pi, P, a.iter - 1, gb,
+                                                 int(blockIdx.x) == P.block0);
   __syncthreads();
+  // update / init coordinates (coalesced over d) and stage the tile
 #pragma unroll
   for (int e = tid; e < kPsoTile * D; e += kPsoTileThreads) {
     const int j = e / D, d = e % D;
     const int64_t sl = base + j;
-    if (sl < P.n_starts) {
-      const int64_t s = P.start0 + sl;
-      pso_coord(a, P, pi, D, s, d);
-      xs[d * S::XS + j] = a.X[s * D + d];
-    }
+    if (sl < P.n_starts)
+      xs[d * S::XS + j] = pso_coord(a, P, D, P.start0 + sl, d, gb);
   }
   __syncthreads();
 
-  const double* Cf = P.aux_frag;
   for (int u = w; u < S::U; u += NW) {
-    const mfma_acc_t acc = cs_unit<D>(Cf, xs, u, li, lk);
+    const mfma_acc_t acc = cs_unit<D>(P.aux_frag, xs, u, li, lk);
     double fp = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -255,11 +295,10 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   __syncthreads();
 
   // f_i, pBest on strict improvement
-  double v = __builtin_nan("");
-  int64_t vi = int64_t(1) << 62;
   if (tid < kPsoTile) {
     const int64_t sl = base + tid;
     int imp = 0;
+    double v = __builtin_nan("");
     if (sl < P.n_starts) {
       double t = 0.0;
       for (int u = 0; u < S::U; ++u)
@@ -268,22 +307,66 @@ pso_gaussian_mfma_kernel(PsoArgs a)
       const int64_t s = P.start0 + sl;
       if (a.iter == 0) {
         a.pbv[s] = fv;
-      } else if (fv < a.pbv[s]) {
-        a.pbv[s] = fv;
-        imp = 1;
+        v = fv;
+      } else {
+        v = a.pbv[s];
+        if (fv < v) {
+          a.pbv[s] = fv;
+          v = fv;
+          imp = 1;
+        }
       }
-      v = a.pbv[s];
-      vi = sl;
     }
+    pbv[tid] = v;
     improved[tid] = imp;
   }
   __syncthreads();
+  if (tid == 0) { // best particle of the workgroup (NaN loses, ties to the lowest index)
+    double bv = __builtin_nan("");
+    int64_t bi = int64_t(1) << 62;
+    int bj = 0;
+    for (int j = 0; j < kPsoTile; ++j)
+      if (base + j < P.n_starts && better(pbv[j], base + j, bv, bi)) {
+        bv = pbv[j];
+        bi = base + j;
+        bj = j;
+      }
+    best_j = bj;
+    pso_publish_best(a, bv, bi);
+  }
+  __syncthreads();
+  // pBest rows of improved particles, and the best particle's coordinates:
+  // its pBest is the staged position if it was set now, else its pBest row
+  // (not written by this launch)
+  double* px = a.part_x + (int64_t(a.iter & 1) * a.n_blocks + blockIdx.x) * D;
   for (int e = tid; e < kPsoTile * D; e += kPsoTileThreads) {
     const int j = e / D, d = e % D;
-    if (improved[j])
-      a.pbx[(P.start0 + base + j) * D + d] = xs[d * S::XS + j];
+    const int64_t s = P.start0 + base + j;
+    if (base + j < P.n_starts) {
+      if (improved[j])
+        a.pbx[s * D + d] = xs[d * S::XS + j];
+      if (j == best_j)
+        px[d] = (improved[j] || a.iter == 0) ? xs[d * S::XS + j] : a.pbx[s * D + d];
+    }
   }
-  publish_best<kPsoTileThreads, kPsoTile, D>(a, pi, P, v, vi);
+}
+
+// gBest after a problem's last iteration, for callers that want it: one
+// workgroup per problem, writing the gbx row and Control::gbest_val.
+template <int NT, int PPB, int D>
+__global__ __launch_bounds__(NT) void
+pso_final_kernel(PsoArgs a)
+{
+  __shared__ double gb[D];
+  const int pi = blockIdx.x;
+  const ProblemDesc P = a.probs[pi];
+  if (P.pso_iter < 0 || P.n_starts == 0)
+    return;
+  const double val = pso_fold<NT, PPB, D>(a, pi, P, P.pso_iter, gb, false);
+  for (int d = threadIdx.x; d < D; d += NT)
+    a.gbx[int64_t(pi) * D + d] = gb[d];
+  if (threadIdx.x == 0)
+    a.ctl[pi].gbest_val = val;
 }
 
 } // namespace zeus::dev

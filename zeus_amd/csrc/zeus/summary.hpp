@@ -25,6 +25,15 @@ init_controls_kernel(Control* c, int n, double* gbx, int64_t n_gbx)
     gbx[k] = 0.0;
 }
 
+// Zero n 32-bit words (a fill kernel: capturable, no memset node).
+static __global__ void
+zero_words_kernel(uint32_t* p, int n)
+{
+  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = 0u;
+}
+
 // Cs = (C + C^T) / 2 for a Gaussian problem (same quadratic form), row
 // major, and the same matrix in MFMA A-operand order: fragment (mb, kb) is
 // 64 consecutive doubles, lane l holding Cs[16 mb + l % 16][4 kb + l / 16]

@@ -103,8 +103,9 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     const UserProvider<F> prov{reinterpret_cast<const F*>(fbuf.data())};
 
     HipBuffer<double> X(pso ? n * D : 0), V(pso ? n * D : 0), pbx(pso ? n * D : 0),
-      pbv(pso ? n : 0), gbx(D), part_val(nblk);
-    HipBuffer<int64_t> part_idx(nblk), bxo(1);
+      pbv(pso ? n : 0), gbx(D), part_val(2 * nblk), part_x(pso ? 2 * int64_t(nblk) * D : 0),
+      gstate(2 * (D + 1));
+    HipBuffer<int64_t> part_idx(2 * nblk), bxo(1);
     HipBuffer<int32_t> bprob(nblk);
     std::vector<int32_t> zeros(nblk, 0);
     abi_check(zeus_memcpy_h2d(bprob.data(), zeros.data(), 4 * nblk), "upload");
@@ -167,8 +168,9 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     hip_check(hipEventCreate(&e1), "event");
     init_controls_kernel<<<1, 256, 0, s>>>(ctl.data(), 1, gbx.data(), D);
     if (pso) {
-      PsoArgs pa{dP.data(), 1, 0, ctl.data(), X.data(), V.data(), pbx.data(), pbv.data(),
-                 gbx.data(), part_val.data(), part_idx.data(), bprob.data()};
+      PsoArgs pa{dP.data(),       1,           0,           nblk,           ctl.data(),
+                 X.data(),        V.data(),    pbx.data(),  pbv.data(),     gbx.data(),
+                 part_val.data(), part_idx.data(), part_x.data(), gstate.data(), bprob.data()};
       for (int t = 0; t <= PSO_ITER; ++t) {
         pa.iter = t;
         pso_lane_kernel<F, D, UserProvider<F>><<<nblk, 256, 0, s>>>(pa, prov);

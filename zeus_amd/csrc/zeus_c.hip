@@ -139,6 +139,7 @@ ensure_device()
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return ZEUS_E_NODEVICE; // code objects are built for gfx950 only
   g_ds.n_cu = prop.multiProcessorCount;
+
   if (hipStreamCreateWithFlags(&g_ds.stream, hipStreamNonBlocking) != hipSuccess)
     return ZEUS_E_LAUNCH;
   for (auto& e : g_ds.ev)
@@ -287,8 +288,10 @@ struct ClassPlan {
   ProblemDesc* d_desc = nullptr;
   int32_t* d_block_problem = nullptr;
   double *X = nullptr, *V = nullptr, *pbx = nullptr, *pbv = nullptr, *gbx = nullptr;
-  double* part_val = nullptr;
+  double* part_val = nullptr;  // PSO partials, [2][pso_blocks] (iteration parity)
   int64_t* part_idx = nullptr;
+  double* part_x = nullptr;    // [2][pso_blocks][D]
+  double* gstate = nullptr;    // per problem [2][1 + D]
   StartOut out{};
   double* aux = nullptr; // Gaussian Cs, one D x D block per problem
   double* h_slab = nullptr;
@@ -311,8 +314,10 @@ carve_class(Carver& cv, ClassPlan& c)
   c.pbx = cv.take<double>(pso ? n * D : 1);
   c.pbv = cv.take<double>(pso ? n : 1);
   c.gbx = cv.take<double>(np * D);
-  c.part_val = cv.take<double>(nb);
-  c.part_idx = cv.take<int64_t>(nb);
+  c.part_val = cv.take<double>(2 * nb);
+  c.part_idx = cv.take<int64_t>(2 * nb);
+  c.part_x = cv.take<double>(pso ? 2 * nb * D : 1);
+  c.gstate = cv.take<double>(pso ? np * 2 * (D + 1) : 1);
   c.out.status = cv.take<int32_t>(n);
   c.out.iter = cv.take<int32_t>(n);
   c.out.ls_evals = cv.take<int32_t>(n);
@@ -324,6 +329,14 @@ carve_class(Carver& cv, ClassPlan& c)
   c.aux = cv.take<double>(c.ops->objective == obj::kGaussian ? np * dev::gaussian_aux_doubles(D)
                                                               : 1);
   c.h_slab = cv.take<double>(size_t(std::max<int64_t>(c.slab, 1)));
+
+}
+
+PsoArgs
+pso_args(const ClassPlan& c, Control* ctl)
+{
+  return PsoArgs{c.d_desc, int32_t(c.probs.size()), 0, c.pso_blocks, ctl, c.X, c.V, c.pbx,
+                 c.pbv, c.gbx, c.part_val, c.part_idx, c.part_x, c.gstate, c.d_block_problem};
 }
 
 hipError_t
@@ -344,8 +357,7 @@ launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
       return e;
   }
   if (c.max_pso_iter >= 0 && c.pso_blocks > 0) {
-    PsoArgs pa{c.d_desc, np, 0, ctl, c.X, c.V, c.pbx, c.pbv, c.gbx,
-               c.part_val, c.part_idx, c.d_block_problem};
+    PsoArgs pa = pso_args(c, ctl);
     for (int it = 0; it <= c.max_pso_iter; ++it) {
       pa.iter = it;
       if ((e = c.ops->pso(pa, c.pso_blocks, st)) != hipSuccess)
@@ -882,6 +894,8 @@ zeus_pso_run(const zeus_problem* prob, double* pbest_x, double* pbest_val, doubl
   }
   if (e == hipSuccess)
     e = launch_class(c, d_ctl, 0, st);
+  if (e == hipSuccess)
+    e = c.ops->pso_final(pso_args(c, d_ctl), 1, st);
   if (e == hipSuccess && gbest_val)
     e = hipMemcpyAsync(gbest_val, &d_ctl->gbest_val, sizeof(double), hipMemcpyDeviceToDevice,
                        st);
