@@ -67,19 +67,24 @@ gaussian_aux_doubles(int D)
 // reference's launchReduction, bfgs_common.cuh:280-319, with NaN skipped
 // and ties to the lowest index) plus the status counts of
 // util.hpp:151-200 and the work counters used for the roofline.
-static __global__ __launch_bounds__(256) void
+constexpr int kSummaryThreads = 1024;
+
+static __global__ __launch_bounds__(kSummaryThreads) void
 summary_kernel(const ProblemDesc* probs, Control* ctl, StartOut out, double* best_x,
                const int64_t* best_x_off)
 {
-  __shared__ double sv[4];
-  __shared__ int64_t si[4];
-  __shared__ int64_t sc[4][8];
+  constexpr int NW = kSummaryThreads / kWave;
+  __shared__ double sv[NW];
+  __shared__ int64_t si[NW];
+  __shared__ int64_t sc[NW][8];
+  __shared__ int64_t sbest;
   const int pi = blockIdx.x;
   const ProblemDesc P = probs[pi];
   double v = __builtin_nan("");
   int64_t idx = int64_t(1) << 62;
   int64_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t k = threadIdx.x; k < P.n_starts; k += 256) {
+#pragma unroll 4
+  for (int64_t k = threadIdx.x; k < P.n_starts; k += kSummaryThreads) {
     const int64_t s = P.start0 + k;
     const double f = out.fval[s];
     if (better(f, k, v, idx)) {
@@ -96,7 +101,7 @@ summary_kernel(const ProblemDesc* probs, Control* ctl, StartOut out, double* bes
     cnt[6] += out.grads[s];
     cnt[7] += out.near[s];
   }
-  block_argmin<256>(v, idx, sv, si);
+  block_argmin<kSummaryThreads>(v, idx, sv, si);
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     int64_t t = cnt[c];
@@ -107,11 +112,11 @@ summary_kernel(const ProblemDesc* probs, Control* ctl, StartOut out, double* bes
       sc[threadIdx.x >> 6][c] = t;
   }
   __syncthreads();
-  __shared__ int64_t sbest;
   if (threadIdx.x == 0) {
-    int64_t tot[8];
-    for (int c = 0; c < 8; ++c)
-      tot[c] = sc[0][c] + sc[1][c] + sc[2][c] + sc[3][c];
+    int64_t tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int w = 0; w < NW; ++w)
+      for (int c = 0; c < 8; ++c)
+        tot[c] += sc[w][c];
     ctl[pi].actual = int32_t(tot[7]);
     const int64_t b = P.n_starts > 0 ? (idx < P.n_starts ? idx : 0) : -1;
     ctl[pi].best_idx = int32_t(b);
@@ -133,7 +138,7 @@ summary_kernel(const ProblemDesc* probs, Control* ctl, StartOut out, double* bes
   }
   __syncthreads();
   if (sbest >= 0)
-    for (int d = threadIdx.x; d < P.dim; d += 256)
+    for (int d = threadIdx.x; d < P.dim; d += kSummaryThreads)
       best_x[best_x_off[pi] + d] = out.x[(P.start0 + sbest) * P.dim + d];
 }
 

@@ -185,7 +185,7 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
       bfgs_thread_kernel<F, D, G, UserProvider<F>><<<nblk, 256, 0, s>>>(a, prov);
     hip_check(hipGetLastError(), "BFGS launch");
     hip_check(hipEventRecord(e1, s), "event");
-    summary_kernel<<<1, 256, 0, s>>>(dP.data(), ctl.data(), out, bx.data(), bxo.data());
+    summary_kernel<<<1, kSummaryThreads, 0, s>>>(dP.data(), ctl.data(), out, bx.data(), bxo.data());
     hip_check(hipGetLastError(), "summary launch");
     hip_check(hipStreamSynchronize(s), "BFGS runtime");
     float ms = 0.f;

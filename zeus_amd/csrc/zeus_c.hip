@@ -91,6 +91,8 @@ struct DeviceState {
   hipEvent_t ev[6] = {};
   void* arena = nullptr;
   size_t arena_bytes = 0;
+  void* pinned = nullptr; // host mirror of the arena's upload/download regions
+  size_t pinned_bytes = 0;
   std::map<TuneKey, TuneVal> tuned; // autotuner results (only tuned choices)
   std::map<std::tuple<int, int, int>, TuneVal> last; // last shape used per class and gradient
   struct Graph {
@@ -171,6 +173,27 @@ arena_reserve(size_t bytes)
     return ZEUS_E_ALLOC;
   g_ds.arena = p;
   g_ds.arena_bytes = want;
+  return ZEUS_OK;
+}
+
+// Grow-only pinned host buffer (a fresh allocation: pinned memory is
+// allocated, never registered after the fact).
+zeus_status_t
+pinned_reserve(size_t bytes)
+{
+  if (bytes <= g_ds.pinned_bytes)
+    return ZEUS_OK;
+  if (g_ds.pinned) {
+    (void)hipHostFree(g_ds.pinned);
+    g_ds.pinned = nullptr;
+    g_ds.pinned_bytes = 0;
+  }
+  const size_t want = bytes + bytes / 4 + 4096;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess)
+    return ZEUS_E_ALLOC;
+  g_ds.pinned = p;
+  g_ds.pinned_bytes = want;
   return ZEUS_OK;
 }
 
@@ -301,13 +324,19 @@ struct ClassPlan {
   std::vector<int32_t> block_problem;
 };
 
+// Host-written inputs of a class (uploaded in one copy with the others).
+void
+carve_upload(Carver& cv, ClassPlan& c)
+{
+  c.d_desc = cv.take<ProblemDesc>(c.probs.size());
+  c.d_block_problem = cv.take<int32_t>(size_t(std::max(c.pso_blocks, 1)));
+}
+
 void
 carve_class(Carver& cv, ClassPlan& c)
 {
   const int D = c.ops->dim;
   const size_t n = size_t(c.total), np = c.probs.size(), nb = size_t(std::max(c.pso_blocks, 1));
-  c.d_desc = cv.take<ProblemDesc>(np);
-  c.d_block_problem = cv.take<int32_t>(nb);
   const bool pso = c.max_pso_iter >= 0;
   c.X = cv.take<double>(pso ? n * D : 1);
   c.V = cv.take<double>(pso ? n * D : 1);
@@ -554,12 +583,21 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   }
 
   // ---- device memory ------------------------------------------------------
+  // Arena layout: [inputs written by the host | results read back | the
+  // rest].  The first two regions are mirrored in pinned host memory, so a
+  // call makes one upload copy and one download copy.
+  size_t up_end = 0, dn_begin = 0, dn_end = 0;
   auto layout = [&](Carver& cv, Control** ctl, double** bx, int64_t** bxo) {
     for (auto& c : classes)
-      carve_class(cv, c);
-    *ctl = cv.take<Control>(size_t(ctl_total));
-    *bx = cv.take<double>(size_t(dims_all));
+      carve_upload(cv, c);
     *bxo = cv.take<int64_t>(size_t(ctl_total));
+    up_end = cv.off;
+    *ctl = cv.take<Control>(size_t(ctl_total));
+    dn_begin = reinterpret_cast<char*>(*ctl) - cv.base;
+    *bx = cv.take<double>(size_t(dims_all));
+    dn_end = cv.off;
+    for (auto& c : classes)
+      carve_class(cv, c);
   };
   Control* d_ctl;
   double* d_bx;
@@ -567,7 +605,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   {
     Carver m{nullptr};
     layout(m, &d_ctl, &d_bx, &d_bxo);
-    if ((rc = arena_reserve(m.off)) != ZEUS_OK) {
+    if ((rc = arena_reserve(m.off)) != ZEUS_OK || (rc = pinned_reserve(dn_end)) != ZEUS_OK) {
       for (int i = 0; i < n_probs; ++i)
         results[i].status = ZEUS_MALLOC;
       return rc;
@@ -575,6 +613,11 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     Carver cv{static_cast<char*>(g_ds.arena)};
     layout(cv, &d_ctl, &d_bx, &d_bxo);
   }
+  // host mirror of the first two regions, at the same offsets
+  char* const pin = static_cast<char*>(g_ds.pinned);
+  auto mirror = [&](const void* dev_ptr) {
+    return pin + (static_cast<const char*>(dev_ptr) - static_cast<char*>(g_ds.arena));
+  };
   // problems that run PSO start their BFGS from pBestX
   for (auto& c : classes)
     for (auto& d : c.desc) {
@@ -607,17 +650,12 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     return s;
   };
   for (auto& c : classes) {
-    if ((e = hipMemcpyAsync(c.d_desc, c.desc.data(), sizeof(ProblemDesc) * c.desc.size(),
-                            hipMemcpyHostToDevice, st)) != hipSuccess)
-      return fail(e);
-    if (!c.block_problem.empty() &&
-        (e = hipMemcpyAsync(c.d_block_problem, c.block_problem.data(),
-                            sizeof(int32_t) * c.block_problem.size(), hipMemcpyHostToDevice,
-                            st)) != hipSuccess)
-      return fail(e);
+    std::memcpy(mirror(c.d_desc), c.desc.data(), sizeof(ProblemDesc) * c.desc.size());
+    std::memcpy(mirror(c.d_block_problem), c.block_problem.data(),
+                sizeof(int32_t) * c.block_problem.size());
   }
-  if ((e = hipMemcpyAsync(d_bxo, bxo.data(), sizeof(int64_t) * bxo.size(),
-                          hipMemcpyHostToDevice, st)) != hipSuccess)
+  std::memcpy(mirror(d_bxo), bxo.data(), sizeof(int64_t) * bxo.size());
+  if ((e = hipMemcpyAsync(g_ds.arena, pin, up_end, hipMemcpyHostToDevice, st)) != hipSuccess)
     return fail(e);
 
   // ---- launch sequence ----------------------------------------------------
@@ -651,8 +689,8 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     if (timed)
       (void)hipEventRecord(g_ds.ev[2], st);
     for (auto& c : classes) {
-      dev::summary_kernel<<<int(c.probs.size()), 256, 0, st>>>(c.d_desc, d_ctl + c.ctl0, c.out,
-                                                               d_bx, d_bxo + c.ctl0);
+      dev::summary_kernel<<<int(c.probs.size()), dev::kSummaryThreads, 0, st>>>(
+        c.d_desc, d_ctl + c.ctl0, c.out, d_bx, d_bxo + c.ctl0);
     }
     return hipGetLastError();
   };
@@ -703,16 +741,13 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     return fail(e);
   }
 
-  std::vector<Control> h_ctl(ctl_total);
-  if ((e = hipMemcpyAsync(h_ctl.data(), d_ctl, sizeof(Control) * ctl_total,
-                          hipMemcpyDeviceToHost, st)) != hipSuccess)
-    return fail(e);
-  std::vector<double> h_bx(dims_all);
-  if ((e = hipMemcpyAsync(h_bx.data(), d_bx, sizeof(double) * dims_all, hipMemcpyDeviceToHost,
-                          st)) != hipSuccess)
+  if ((e = hipMemcpyAsync(pin + dn_begin, static_cast<char*>(g_ds.arena) + dn_begin,
+                          dn_end - dn_begin, hipMemcpyDeviceToHost, st)) != hipSuccess)
     return fail(e);
   if ((e = hipStreamSynchronize(st)) != hipSuccess)
     return fail(e);
+  const Control* h_ctl = reinterpret_cast<const Control*>(mirror(d_ctl));
+  const double* h_bx = reinterpret_cast<const double*>(mirror(d_bx));
 
   float ms_pso = 0.f, ms_opt = 0.f;
   if (graphed) { // one replay: its device time (PSO, BFGS and summary) in ms_opt
@@ -784,7 +819,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     r.ms_opt = ms_opt;
     r.ms_total = ms_total;
     if (best_x)
-      std::memcpy(best_x + bx_user[i], h_bx.data() + bx_user[i], sizeof(double) * probs[i].dim);
+      std::memcpy(best_x + bx_user[i], h_bx + bx_user[i], sizeof(double) * probs[i].dim);
   }
   return ZEUS_OK;
 }
@@ -864,6 +899,7 @@ zeus_pso_run(const zeus_problem* prob, double* pbest_x, double* pbest_val, doubl
   c.max_pso_iter = prob->pso_iter;
   Control* d_ctl;
   auto layout = [&](Carver& cv) {
+    carve_upload(cv, c);
     carve_class(cv, c);
     d_ctl = cv.take<Control>(1);
   };
