@@ -34,7 +34,7 @@ if ROOT not in sys.path:
 METRIC = "converged BFGS starts/sec @tol 1e-8, 150-D Gaussian: adjoint vs dual-AD on 1 GPU"
 D, N, PSO_ITER, MAX_ITER, REQUIRED, TOL, SEED = 150, 4096, 10, 10000, 100, 1e-8, 42
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gaussian150_bfgs_group_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gaussian150_bfgs_tile_traffic.md")
 
 
 def gauss_matrix():
