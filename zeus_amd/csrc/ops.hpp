@@ -23,6 +23,10 @@ template <class F, int D>
 struct Ops {
   static constexpr bool kMfmaPso = F::id == obj::kGaussian && D >= 16;
   static constexpr bool kThreadOk = F::id != obj::kMlp && D <= 64;
+  // workgroup sizes of the group form (the MLP's cooperative evaluator puts
+  // one point per lane and one hidden-to-hidden weight per lane: 256)
+  static constexpr int kNGroup = F::id == obj::kMlp ? 1 : group_nt_count(D);
+  static constexpr int nt_at(int k) { return F::id == obj::kMlp ? 256 : group_nt_at(D, k); }
   // tile form: Gaussian classes whose PSO already runs in tiles of 16
   static constexpr bool kTileOk = kMfmaPso;
 
@@ -91,14 +95,14 @@ struct Ops {
   {
     if (fault_take_launch())
       return hipErrorLaunchFailure;
-    if (nt == group_nt_at(D, 0))
-      return group_nt<group_nt_at(D, 0)>(a, gradient, grid, s);
-    if constexpr (group_nt_count(D) > 1)
-      if (nt == group_nt_at(D, 1))
-        return group_nt<group_nt_at(D, 1)>(a, gradient, grid, s);
-    if constexpr (group_nt_count(D) > 2)
-      if (nt == group_nt_at(D, 2))
-        return group_nt<group_nt_at(D, 2)>(a, gradient, grid, s);
+    if (nt == nt_at(0))
+      return group_nt<nt_at(0)>(a, gradient, grid, s);
+    if constexpr (kNGroup > 1)
+      if (nt == nt_at(1))
+        return group_nt<nt_at(1)>(a, gradient, grid, s);
+    if constexpr (kNGroup > 2)
+      if (nt == nt_at(2))
+        return group_nt<nt_at(2)>(a, gradient, grid, s);
     return hipErrorInvalidConfiguration;
   }
 
@@ -127,14 +131,14 @@ struct Ops {
 
   static int group_blocks_per_cu(int gradient, int nt)
   {
-    if (nt == group_nt_at(D, 0))
-      return occ<group_nt_at(D, 0)>(gradient);
-    if constexpr (group_nt_count(D) > 1)
-      if (nt == group_nt_at(D, 1))
-        return occ<group_nt_at(D, 1)>(gradient);
-    if constexpr (group_nt_count(D) > 2)
-      if (nt == group_nt_at(D, 2))
-        return occ<group_nt_at(D, 2)>(gradient);
+    if (nt == nt_at(0))
+      return occ<nt_at(0)>(gradient);
+    if constexpr (kNGroup > 1)
+      if (nt == nt_at(1))
+        return occ<nt_at(1)>(gradient);
+    if constexpr (kNGroup > 2)
+      if (nt == nt_at(2))
+        return occ<nt_at(2)>(gradient);
     return 1;
   }
 
@@ -145,14 +149,14 @@ struct Ops {
   }
   static int64_t group_slab_per_block(int nt)
   {
-    if (nt == group_nt_at(D, 0))
-      return slab_doubles<group_nt_at(D, 0)>();
-    if constexpr (group_nt_count(D) > 1)
-      if (nt == group_nt_at(D, 1))
-        return slab_doubles<group_nt_at(D, 1)>();
-    if constexpr (group_nt_count(D) > 2)
-      if (nt == group_nt_at(D, 2))
-        return slab_doubles<group_nt_at(D, 2)>();
+    if (nt == nt_at(0))
+      return slab_doubles<nt_at(0)>();
+    if constexpr (kNGroup > 1)
+      if (nt == nt_at(1))
+        return slab_doubles<nt_at(1)>();
+    if constexpr (kNGroup > 2)
+      if (nt == nt_at(2))
+        return slab_doubles<nt_at(2)>();
     return 0;
   }
 
@@ -210,9 +214,9 @@ struct Ops {
       o.thread_ok = kThreadOk;
       o.bfgs_thread = &bfgs_thread;
       o.thread_slab_per_start = D <= dev::kThreadRegMaxD ? 0 : int64_t(D) * (D + 1) / 2;
-      o.n_group = group_nt_count(D);
+      o.n_group = kNGroup;
       for (int k = 0; k < o.n_group; ++k)
-        o.group_nt[k] = group_nt_at(D, k);
+        o.group_nt[k] = nt_at(k);
       o.bfgs_group = &bfgs_group;
       o.group_blocks_per_cu = &group_blocks_per_cu;
       o.group_slab_per_block = &group_slab_per_block;

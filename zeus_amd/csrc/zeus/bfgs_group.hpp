@@ -272,6 +272,152 @@ struct GroupEvalLogLik {
   }
 };
 
+// ---------------------------------------------------------------------------
+// MLP (1-16-16-1 tanh, least squares over m points): points spread over the
+// lanes.  Value: each lane's squared residuals, one workgroup sum.  Adjoint
+// gradient: each lane runs its point's forward and reverse sweep
+// (activations recomputed, no tape), then the sums over points are formed
+// in a fixed order: the 65 per-neuron terms (output weights and bias,
+// hidden biases, first-layer weights) by wavefront sums and a per-wavefront
+// fold, the 256 hidden-to-hidden weights as dot products over the points
+// staged in LDS (lane k*16 + j owns weight (k, j)).  The dual gradient keeps
+// the reference's D forward passes spread over the lanes.
+template <int NT, Gradient G>
+struct GroupEvalMlp {
+  using F = obj::Mlp;
+  static constexpr int H = F::H, W2 = F::W2, W3 = F::W3;
+  static constexpr int kCols = 4 * H + 1; // w3 (16), b3, b2 (16), w1 (16), b1 (16)
+  static_assert(NT >= H * H, "one lane per hidden-to-hidden weight");
+  GroupScratch sc;
+  ZEUS_D void bind(const ProblemDesc&) {}
+
+  // forward pass of point p: hidden activations and output
+  ZEUS_D ZEUS_FI static double forward(const F& f, const double* th, int p, double (&h1)[H],
+                                       double (&h2)[H])
+  {
+    const double xp = f.P[p];
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      double sum = 0.0;
+      sum += th[2 * j] * xp;
+      sum += th[2 * j + 1];
+      h1[j] = ::tanh(sum);
+    }
+#pragma unroll 4
+    for (int k = 0; k < H; ++k) {
+      const double* w = th + W2 + k * (H + 1);
+      double sum = 0.0;
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+        sum += h1[j] * w[j];
+      sum += w[H];
+      h2[k] = ::tanh(sum);
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < H; ++k)
+      sum += h2[k] * th[W3 + k];
+    sum += th[W3 + H];
+    return sum;
+  }
+
+  ZEUS_D double value(const F& f, const double* th, Block<NT>& blk)
+  {
+    double t = 0.0;
+    for (int p = threadIdx.x; p < f.m; p += NT) {
+      double h1[H], h2[H];
+      const double dd = forward(f, th, p, h1, h2) - f.P[f.m + p];
+      t = ::fma(dd, dd, t);
+    }
+    return blk.sum1(t) / double(f.m);
+  }
+
+  ZEUS_D void gradient(const F& f, const double* th, double* g, bool, Block<NT>& blk)
+  {
+    if constexpr (G == Gradient::Dual) {
+      group_dual_gradient<F, int(F::arity), NT>(f, th, g);
+    } else {
+      __shared__ double h1s[NT][H + 1]; // one point per lane, padded rows
+      __shared__ double d2s[NT][H + 1];
+      __shared__ double colw[NT / kWave][kCols];
+      const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+      const double scale = 2.0 / double(f.m);
+      for (int c = lane; c < kCols; c += kWave)
+        colw[w][c] = 0.0;
+      // per-neuron term c of this lane's point, summed over the wavefront
+      // and added to the wavefront's running total (chunks in order)
+      auto col_add = [&](int c, double v) {
+        const double t = wave_sum(v);
+        if (lane == 0)
+          colw[w][c] += t;
+      };
+      double gw = 0.0; // hidden-to-hidden weight (k, j) = (tid / H, tid % H)
+      for (int c0 = 0; c0 < f.m; c0 += NT) {
+        const int p = c0 + tid;
+        const bool live = p < f.m;
+        double h1[H], h2[H], d2[H];
+        double dy = 0.0;
+        if (live) {
+          dy = scale * (forward(f, th, p, h1, h2) - f.P[f.m + p]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < H; ++j)
+            h1[j] = h2[j] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+          col_add(k, dy * h2[k]);                                  // w3
+          d2[k] = (dy * th[W3 + k]) * (1.0 - h2[k] * h2[k]);
+          col_add(H + 1 + k, d2[k]);                               // b2
+        }
+        col_add(H, dy);                                            // b3
+        const double xp = live ? f.P[p] : 0.0;
+#pragma unroll 4
+        for (int j = 0; j < H; ++j) {
+          double d1 = 0.0;
+#pragma unroll
+          for (int k = 0; k < H; ++k)
+            d1 += d2[k] * th[W2 + k * (H + 1) + j];
+          const double dj = d1 * (1.0 - h1[j] * h1[j]);
+          col_add(2 * H + 1 + j, dj * xp);                         // w1
+          col_add(3 * H + 1 + j, dj);                              // b1
+        }
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+          h1s[tid][j] = h1[j];
+          d2s[tid][j] = d2[j];
+        }
+        __syncthreads();
+        if (tid < H * H) {
+          const int k = tid / H, j = tid % H;
+          const int n = f.m - c0 < NT ? f.m - c0 : NT;
+          for (int q = 0; q < n; ++q)
+            gw = ::fma(d2s[q][k], h1s[q][j], gw);
+        }
+        __syncthreads();
+      }
+      for (int c = tid; c < kCols; c += NT) {
+        double t = 0.0;
+        for (int v = 0; v < NT / kWave; ++v)
+          t += colw[v][c];
+        if (c < H)
+          g[W3 + c] = t;
+        else if (c == H)
+          g[W3 + H] = t;
+        else if (c < 2 * H + 1)
+          g[W2 + (c - H - 1) * (H + 1) + H] = t;
+        else if (c < 3 * H + 1)
+          g[2 * (c - 2 * H - 1)] = t;
+        else
+          g[2 * (c - 3 * H - 1) + 1] = t;
+      }
+      if (tid < H * H)
+        g[W2 + (tid / H) * (H + 1) + tid % H] = gw;
+    }
+    (void)blk;
+  }
+};
+
 // Evaluator per objective type.  Specialisations key on the objective type
 // alone (its dimension parameter is a std::size_t; deducing it through an
 // int parameter would fail and silently select the one-lane fallback).
@@ -302,6 +448,10 @@ struct GroupEvalSelect<obj::Sphere<D>, NT, G> {
 template <int NT, Gradient G>
 struct GroupEvalSelect<obj::LogLikelihood, NT, G> {
   using type = GroupEvalLogLik<NT, G>;
+};
+template <int NT, Gradient G>
+struct GroupEvalSelect<obj::Mlp, NT, G> {
+  using type = GroupEvalMlp<NT, G>;
 };
 
 // ---------------------------------------------------------------------------
