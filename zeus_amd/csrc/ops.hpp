@@ -29,6 +29,11 @@ struct Ops {
   static constexpr int nt_at(int k) { return F::id == obj::kMlp ? 256 : group_nt_at(D, k); }
   // tile form: Gaussian classes whose PSO already runs in tiles of 16
   static constexpr bool kTileOk = kMfmaPso;
+  // PSO with a workgroup per particle: objectives whose one evaluation is
+  // worth a workgroup (the MLP loss over its data points)
+  static constexpr bool kGroupPso = F::id == obj::kMlp;
+  using PsoEval = typename dev::GroupEvalSelect<F, 256, Gradient::Adjoint>::type;
+  static constexpr int kPsoPpb = kMfmaPso ? dev::kPsoTile : kGroupPso ? dev::kPsoGroupPpb : 256;
 
   static hipError_t pso(const dev::PsoArgs& a, int nb, hipStream_t s)
   {
@@ -36,6 +41,9 @@ struct Ops {
       return hipErrorLaunchFailure;
     if constexpr (kMfmaPso)
       dev::pso_gaussian_mfma_kernel<D><<<nb, dev::kPsoTileThreads, 0, s>>>(a);
+    else if constexpr (kGroupPso)
+      dev::pso_group_kernel<F, D, 256, kPsoPpb, PsoEval, dev::BuiltinProvider>
+        <<<nb, 256, 0, s>>>(a, {});
     else
       dev::pso_lane_kernel<F, D, dev::BuiltinProvider><<<nb, 256, 0, s>>>(a, {});
     return hipGetLastError();
@@ -46,10 +54,7 @@ struct Ops {
   {
     if (fault_take_launch())
       return hipErrorLaunchFailure;
-    if constexpr (kMfmaPso)
-      dev::pso_final_kernel<256, dev::kPsoTile, D><<<n_probs, 256, 0, s>>>(a);
-    else
-      dev::pso_final_kernel<256, 256, D><<<n_probs, 256, 0, s>>>(a);
+    dev::pso_final_kernel<256, kPsoPpb, D><<<n_probs, 256, 0, s>>>(a);
     return hipGetLastError();
   }
 
@@ -208,7 +213,7 @@ struct Ops {
       ClassOps o;
       o.objective = F::id;
       o.dim = D;
-      o.pso_ppb = kMfmaPso ? dev::kPsoTile : 256;
+      o.pso_ppb = kPsoPpb;
       o.pso = &pso;
       o.pso_final = &pso_final;
       o.thread_ok = kThreadOk;

@@ -22,6 +22,7 @@
 #pragma once
 
 #include "batch.hpp"
+#include "bfgs_group.hpp"
 #include "cs_product.hpp"
 #include "lane_ops.hpp"
 #include "philox.hpp"
@@ -231,6 +232,74 @@ pso_lane_kernel(PsoArgs a, Prov prov)
   const int64_t bs = P.start0 + s_best;
   double* px = a.part_x + (int64_t(a.iter & 1) * a.n_blocks + blockIdx.x) * D;
   for (int d = threadIdx.x; d < D; d += 256)
+    px[d] = a.pbx[bs * D + d];
+}
+
+// Cooperative form: a workgroup owns PPB particles and evaluates each with
+// all of its lanes (Eval is a workgroup evaluator whose value() needs only
+// the Block reductions, e.g. the MLP loss with one data point per lane).
+// Per particle: new coordinates into LDS, value, pBest on strict
+// improvement; the workgroup's best pBest is then published like the lane
+// form's.
+constexpr int kPsoGroupPpb = 2;
+
+template <class F, int D, int NT, int PPB, class Eval, class Prov>
+__global__ __launch_bounds__(NT) void
+pso_group_kernel(PsoArgs a, Prov prov)
+{
+  __shared__ double gb[D];
+  __shared__ double xs[D];
+  __shared__ double red[8 * (NT / kWave)];
+  __shared__ double stash[D];
+  __shared__ double scalar[4];
+  __shared__ int64_t s_best;
+  const int pi = a.block_problem[blockIdx.x];
+  const ProblemDesc P = a.probs[pi];
+  if (a.iter > P.pso_iter)
+    return;
+  if (a.iter > 0)
+    (void)pso_fold<NT, PPB, D>(a, pi, P, a.iter - 1, gb, int(blockIdx.x) == P.block0);
+  Block<NT> blk{red};
+  Eval ev;
+  ev.sc = GroupScratch{nullptr, stash, scalar};
+  ev.bind(P);
+  decltype(auto) f = prov.template get<F>(P);
+  const int tid = threadIdx.x;
+  double bv = __builtin_nan("");
+  int64_t bi = int64_t(1) << 62;
+  for (int j = 0; j < PPB; ++j) {
+    const int64_t sl = int64_t(blockIdx.x - P.block0) * PPB + j;
+    if (sl >= P.n_starts)
+      break; // uniform across the workgroup
+    const int64_t s = P.start0 + sl;
+    for (int d = tid; d < D; d += NT)
+      xs[d] = pso_coord(a, P, D, s, d, gb);
+    const double old = a.iter == 0 ? 0.0 : a.pbv[s];
+    __syncthreads();
+    const double fv = ev.value(f, xs, blk);
+    const bool imp = a.iter == 0 || fv < old;
+    if (imp) {
+      if (tid == 0)
+        a.pbv[s] = fv;
+      if (a.iter > 0)
+        for (int d = tid; d < D; d += NT)
+          a.pbx[s * D + d] = xs[d];
+    }
+    const double pv = imp ? fv : old;
+    if (better(pv, sl, bv, bi)) {
+      bv = pv;
+      bi = sl;
+    }
+    __syncthreads(); // xs is rewritten by the next particle
+  }
+  if (tid == 0) {
+    pso_publish_best(a, bv, bi);
+    s_best = bi;
+  }
+  __syncthreads();
+  const int64_t bs = P.start0 + s_best;
+  double* px = a.part_x + (int64_t(a.iter & 1) * a.n_blocks + blockIdx.x) * D;
+  for (int d = tid; d < D; d += NT)
     px[d] = a.pbx[bs * D + d];
 }
 
