@@ -169,15 +169,28 @@ struct DistMatrix {
   ZEUS_D ZEUS_FI void pass(const double* x, const double* s, const double* a,
                            bool pending, double* part)
   {
+    if constexpr (!REG) {
+      if (pending)
+        pass_slab<true>(x, s, a, part);
+      else
+        pass_slab<false>(x, s, a, part);
+    } else {
+      pass_regs(x, s, a, pending, part);
+    }
+  }
+
+  ZEUS_D ZEUS_FI void pass_regs(const double* x, const double* s, const double* a,
+                                bool pending, double* part)
+  {
     const int c = threadIdx.x % Lay::CH, r0 = threadIdx.x / Lay::CH;
-#pragma unroll(REG ? 64 : 1)
+#pragma unroll 64
     for (int k = 0; k < Lay::R; ++k) {
       const int row = r0 + k * Lay::RPP;
       if (row < D) {
         const double sr = pending ? s[row] : 0.0;
         const double ar = pending ? a[row] : 0.0;
         double acc = 0.0;
-#pragma unroll(REG ? 64 : 1)
+#pragma unroll 64
         for (int l = 0; l < Lay::L; ++l) {
           const int col = c * Lay::L + l;
           if (col < D) {
@@ -191,6 +204,64 @@ struct DistMatrix {
         }
         part[c * D + row] = acc;
       }
+    }
+  }
+
+  // Slab form of pass(): the same arithmetic in the same order, with the
+  // columns taken in batches of kBatch whose slab loads are all issued before
+  // any of them is used (a branch per element would wait out the memory
+  // latency element by element).  Padding columns (col >= D) are loaded but
+  // neither updated nor summed.
+  template <bool PENDING>
+  ZEUS_D ZEUS_FI void pass_slab(const double* x, const double* s, const double* a, double* part)
+  {
+    constexpr int kBatch = 16;
+    const int c = threadIdx.x % Lay::CH, r0 = threadIdx.x / Lay::CH;
+    double* base = slab + threadIdx.x;
+    for (int k = 0; k < Lay::R; ++k) {
+      const int row = r0 + k * Lay::RPP;
+      if (row >= D)
+        continue;
+      const double sr = PENDING ? s[row] : 0.0;
+      const double ar = PENDING ? a[row] : 0.0;
+      double* hrow = base + int64_t(k) * Lay::L * NT;
+      double acc = 0.0;
+      int l0 = 0;
+      for (; l0 + kBatch <= Lay::L; l0 += kBatch) {
+        double h[kBatch];
+#pragma unroll
+        for (int i = 0; i < kBatch; ++i)
+          h[i] = hrow[int64_t(l0 + i) * NT];
+#pragma unroll
+        for (int i = 0; i < kBatch; ++i) {
+          const int col = c * Lay::L + l0 + i;
+          const bool ok = col < D;
+          const int cc = ok ? col : D - 1;
+          if constexpr (PENDING) {
+            const double hn = h[i] + (sr * a[cc] + ar * s[cc]);
+            h[i] = ok ? hn : h[i];
+          }
+          const double t = ::fma(h[i], x[cc], acc);
+          acc = ok ? t : acc;
+        }
+        if constexpr (PENDING) {
+#pragma unroll
+          for (int i = 0; i < kBatch; ++i)
+            hrow[int64_t(l0 + i) * NT] = h[i];
+        }
+      }
+      for (int l = l0; l < Lay::L; ++l) {
+        const int col = c * Lay::L + l;
+        if (col < D) {
+          double h = hrow[int64_t(l) * NT];
+          if constexpr (PENDING) {
+            h = h + (sr * a[col] + ar * s[col]);
+            hrow[int64_t(l) * NT] = h;
+          }
+          acc = ::fma(h, x[col], acc);
+        }
+      }
+      part[c * D + row] = acc;
     }
   }
 };
