@@ -36,6 +36,7 @@ typedef int32_t zeus_status_t;
 #define ZEUS_E_ARG (-10)        /* invalid argument value                */
 #define ZEUS_E_UNSUPPORTED (-11) /* objective/dimension not instantiated */
 #define ZEUS_E_NODEVICE (-12)   /* no usable gfx950 device               */
+#define ZEUS_E_IO (-13)         /* a file could not be written           */
 
 /* per-start / per-call status codes (reference README.md:27-35) */
 #define ZEUS_UNSET (-1)
@@ -69,6 +70,33 @@ typedef int32_t zeus_status_t;
                               objective evaluated for the tile on FP64 MFMA
                               (Gaussian with dim >= 16, adjoint gradient)  */
 
+/* Trajectory recording of one problem (reference zeus.cuh:62-103,189-239;
+   recording in bfgs_sequential.cuh:100-109 and bfgs_parallel.cuh:155-184).
+   Caller-owned DEVICE buffers, filled by the call: coords NaN, fval and
+   grad NaN, status -1, alpha 0, then for iteration `it` < steps of start
+   `i` < starts (problem-local):
+     coords[(it * dim + d) * starts + i]  the point at the top of iteration it
+     fval[it * starts + i]                its value (best so far)
+     grad[it * starts + i]                its gradient norm
+     status[it * starts + i]              -1, or the final status at the
+                                          iteration it was decided (0 at
+                                          MAX_ITER - 1 for surrendered)
+     alpha[it * starts + i]               the line-search step (NULL: none;
+                                          the reference records it only in
+                                          its parallel form)
+   steps and starts bound the buffers (the reference keeps all MAX_ITER x N).
+   A traced problem runs the per-thread or per-workgroup form (not the tile
+   form) and its call is never replayed from a graph. */
+typedef struct zeus_trajectory {
+  double* coords;  /* steps x dim x starts */
+  double* fval;    /* steps x starts       */
+  double* grad;    /* steps x starts       */
+  int8_t* status;  /* steps x starts       */
+  double* alpha;   /* steps x starts, or NULL */
+  int32_t steps;   /* iterations kept, >= 0 */
+  int32_t starts;  /* starts kept (the first ones), >= 0 */
+} zeus_trajectory;
+
 /* One optimization problem: the arguments of zeus::Zeus for a built-in
    objective.  Replaces the template parameters/arguments of
    include/zeus.cuh:341-358 (f, lower, upper, N, MAX_ITER, PSO_ITER,
@@ -90,6 +118,7 @@ typedef struct zeus_problem {
   const double* x0;    /* optional device start points N x D (skips PSO)   */
   const char* name;    /* fun_name: selects the known minimum for the
                           "actual" count (reference src/util.cpp:11-41)    */
+  const zeus_trajectory* trajectory; /* optional (NULL: not recorded)      */
 } zeus_problem;
 
 typedef struct zeus_options {
@@ -182,6 +211,18 @@ zeus_status_t zeus_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 zeus_status_t zeus_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
 zeus_status_t zeus_memcpy_d2d(void* dst, const void* src, uint64_t bytes);
 zeus_status_t zeus_synchronize(void);
+
+/* Write trajectory HOST arrays (the layout of zeus_trajectory, with
+   starts = n) to `filename`: reference util::writeTrajectoryData
+   (src/utils.cu:126-286).  ".tsv": one row per (start, step) with columns
+   traj step fval grad status alpha x0 .. x{dim-1}, values in %.17e, NaN as
+   "NaN".  ".nc": NetCDF4 is not built in -> ZEUS_E_UNSUPPORTED (the
+   reference's answer when built without NetCDF).  Any other extension ->
+   ZEUS_E_ARG.  alpha may be NULL (written as 0).  I/O failure -> ZEUS_E_IO. */
+zeus_status_t zeus_write_trajectory(const char* filename, const double* coords,
+                                    const double* fval, const double* grad,
+                                    const int8_t* status, const double* alpha,
+                                    int32_t n, int32_t steps, int32_t dim);
 
 /* Name of a status code ("surrendered", "converged", ...). */
 const char* zeus_status_string(int32_t status);

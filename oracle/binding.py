@@ -47,9 +47,16 @@ class Problem(C.Structure):
                 ("n_params", C.c_int64), ("x0", _dp)]
 
 
+class Traj(C.Structure):
+    _fields_ = [("coords", _dp), ("fval", _dp), ("grad", _dp),
+                ("status", C.POINTER(C.c_int8)), ("alpha", _dp), ("steps", C.c_int32),
+                ("starts", C.c_int32)]
+
+
 class Options(C.Structure):
     _fields_ = [("grad_mode", C.c_int), ("update_mode", C.c_int),
-                ("n_threads", C.c_int), ("early_stop", C.c_int)]
+                ("n_threads", C.c_int), ("early_stop", C.c_int),
+                ("traj", C.POINTER(Traj))]
 
 
 class Best(C.Structure):
@@ -229,6 +236,7 @@ class RunResult:
     grads: np.ndarray
     best: dict
     best_x: np.ndarray
+    trajectory: dict | None = None
 
 
 def _problem(objective, dim, lower, upper, n_starts, required, max_iter, pso_iter,
@@ -240,12 +248,25 @@ def _problem(objective, dim, lower, upper, n_starts, required, max_iter, pso_ite
 
 def run(objective, dim, lower, upper, n_starts, max_iter, pso_iter, required, tol,
         seed, *, stream=0, params=None, x0=None, grad=GRAD_ADJOINT,
-        update_mode=UPDATE_RANK2, threads=0, early_stop=True) -> RunResult:
+        update_mode=UPDATE_RANK2, threads=0, early_stop=True, trajectory=None,
+        trajectory_alpha=True) -> RunResult:
+    """trajectory=(steps, starts) records the BFGS steps (orc_traj layout)."""
     params = None if params is None else _f64(params)
     x0 = None if x0 is None else _f64(x0).reshape(n_starts, dim)
     P = _problem(objective, dim, lower, upper, n_starts, required, max_iter,
                  pso_iter, seed, stream, tol, params, x0)
-    O = Options(grad, update_mode, threads, 1 if early_stop else 0)
+    tarr = None
+    tptr = None
+    if trajectory is not None:
+        T_, S_ = int(trajectory[0]), int(trajectory[1])
+        tarr = dict(coords=np.zeros((T_, dim, S_)), fval=np.zeros((T_, S_)),
+                    grad=np.zeros((T_, S_)), status=np.zeros((T_, S_), np.int8),
+                    alpha=np.zeros((T_, S_)))
+        tr = Traj(_ptr(tarr["coords"]), _ptr(tarr["fval"]), _ptr(tarr["grad"]),
+                  tarr["status"].ctypes.data_as(C.POINTER(C.c_int8)),
+                  _ptr(tarr["alpha"]) if trajectory_alpha else None, T_, S_)
+        tptr = C.pointer(tr)
+    O = Options(grad, update_mode, threads, 1 if early_stop else 0, tptr)
     N = n_starts
     st = np.zeros(N, np.int32)
     it = np.zeros(N, np.int32)
@@ -264,7 +285,7 @@ def run(objective, dim, lower, upper, n_starts, max_iter, pso_iter, required, to
         raise RuntimeError(f"orc_run failed: {rc}")
     best = dict(idx=B.idx, status=B.status, iter=B.iter, fval=B.fval, gnorm=B.gnorm,
                 claimed=B.claimed, stopped=B.stopped, surrendered=B.surrendered)
-    return RunResult(st, it, fv, gn, xs, ev, gr, best, bx)
+    return RunResult(st, it, fv, gn, xs, ev, gr, best, bx, tarr)
 
 
 def pso(objective, dim, lower, upper, n_starts, pso_iter, seed, *, stream=0,

@@ -943,6 +943,15 @@ grad_eval(const orc_objective* f, int mode, const double* x, double* g)
     (void)orc_value_grad_adjoint(f, x, g);
 }
 
+/* Slot of start s at iteration it in a trajectory, or -1 if not kept. */
+static int64_t
+traj_slot(const orc_traj* T, int64_t s, int it)
+{
+  if (!T || !T->fval || s >= T->starts || it < 0 || it >= T->steps)
+    return -1;
+  return (int64_t)it * T->starts + s;
+}
+
 static void
 run_start(const orc_problem* P, const orc_options* O, int64_t s,
           const double* starts, orc_ctx* ctx, start_out* out, double* xo)
@@ -973,7 +982,16 @@ run_start(const orc_problem* P, const orc_options* O, int64_t s,
   double fval = 333777.0, gnorm = 69.0;
   int iter;
   int done = 0;
+  const orc_traj* T = O->traj;
   for (iter = 0; iter < P->max_iter; ++iter) {
+    const int64_t to = traj_slot(T, s, iter);
+    if (to >= 0) { /* bfgs_sequential.cuh:100-109 */
+      for (int d = 0; d < D; ++d)
+        T->coords[((int64_t)iter * D + d) * T->starts + s] = x[d];
+      T->fval[to] = bestVal;
+      T->grad[to] = orc_norm(g, D);
+      T->status[to] = -1;
+    }
     if (O->early_stop && atomic_load(&ctx->stop_flag) != 0) {
       status = 2;
       fval = orc_value(&f, x);
@@ -986,6 +1004,8 @@ run_start(const orc_problem* P, const orc_options* O, int64_t s,
     double facc;
     double alpha = orc_line_search(&f, bestVal, x, p, g, D, &ne, &facc);
     evals += ne;
+    if (to >= 0 && T->alpha) /* bfgs_parallel.cuh:182-184 */
+      T->alpha[to] = alpha;
     for (int i = 0; i < D; ++i) {
       xn[i] = fma(alpha, p[i], x[i]);
       dx[i] = xn[i] - x[i];
@@ -1030,6 +1050,11 @@ run_start(const orc_problem* P, const orc_options* O, int64_t s,
     status = 0;
     fval = orc_value(&f, x);
     gnorm = orc_norm(g, D);
+  }
+  { /* final status at the deciding iteration (bfgs_common.cuh:108-238) */
+    const int64_t to = traj_slot(T, s, status == 0 ? P->max_iter - 1 : iter);
+    if (to >= 0)
+      T->status[to] = (int8_t)status;
   }
   out->status = status;
   out->iter = iter;
@@ -1089,6 +1114,19 @@ orc_run(const orc_problem* P, const orc_options* O, int32_t* status,
     starts = pso_x;
   }
 
+  if (O->traj && O->traj->fval) { /* zeus.cuh:77-103 */
+    const orc_traj* T = O->traj;
+    const int64_t n = (int64_t)T->steps * T->starts;
+    for (int64_t i = 0; i < n * D; ++i)
+      T->coords[i] = NAN;
+    for (int64_t i = 0; i < n; ++i) {
+      T->fval[i] = NAN;
+      T->grad[i] = NAN;
+      T->status[i] = -1;
+      if (T->alpha)
+        T->alpha[i] = 0.0;
+    }
+  }
   start_out* outs = (start_out*)malloc(sizeof(start_out) * (size_t)N);
   double* xs = (double*)malloc(sizeof(double) * (size_t)N * D);
   orc_ctx ctx;

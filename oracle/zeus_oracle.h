@@ -117,11 +117,26 @@ typedef struct {
   const double* x0; /* optional n_starts x dim start points (skips PSO)  */
 } orc_problem;
 
+/* Trajectory recording (include/zeus.cuh:62-103, bfgs_sequential.cuh:
+   100-109, bfgs_parallel.cuh:155-184, bfgs_common.cuh:108-238): for step
+   it < steps of start i < starts, coords[(it*dim + d)*starts + i],
+   fval/grad/status/alpha[it*starts + i].  orc_run fills NaN / -1 / 0 first.
+   alpha may be NULL. */
+typedef struct {
+  double* coords;
+  double* fval;
+  double* grad;
+  int8_t* status;
+  double* alpha;
+  int32_t steps, starts;
+} orc_traj;
+
 typedef struct {
   int grad_mode;   /* ORC_GRAD_*                            */
   int update_mode; /* ORC_UPDATE_*                          */
   int n_threads;   /* OpenMP threads, 0 = runtime default   */
   int early_stop;  /* 0: ignore required (no stop flag)     */
+  const orc_traj* traj; /* NULL: no recording               */
 } orc_options;
 
 typedef struct {

@@ -10,6 +10,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -221,6 +222,12 @@ gpu_cases()
     auto r = zeus::Zeus(UserRosenbrock<2>{}, -5.0, 5.0, 512, 10000, 10, 10, "rosenbrock", 1e-8,
                         42, 0, false);
     print_result("rosenbrock2_thread_early", r);
+  }
+  // trajectory file (zeus.cuh:189-239), per-thread form, bounded to 50 steps
+  if (const char* path = std::getenv("ZEUS_FRONTEND_TRAJ")) {
+    auto r = zeus::Zeus(UserRosenbrock<2>{}, -5.0, 5.0, 32, 300, 3, 33, "rosenbrock", 1e-8, 42,
+                        0, false, zeus::PRNGType::XORWOW, path, 0, zeus::Gradient::Dual, 50);
+    print_result("rosenbrock2_thread_traj", r);
   }
   std::printf("{\"case\": \"checks\", \"failures\": %d}\n", failures);
   return failures == 0 ? 0 : 1;

@@ -66,9 +66,10 @@ def test_zero_starts_returns_default_result():
     assert list(r.coordinates) == [0.0, 0.0]
 
 
-def test_trajectory_file_is_refused_loudly():
-    with pytest.raises(NotImplementedError):
-        z.Zeus(z.Rosenbrock(2), -5, 5, 4, 10, 1, 1, "r", 1e-8, 1, 0, trajectory_file="t.tsv")
+def test_trajectory_argument_validation_needs_no_device():
+    pr = z.Problem(z.Rosenbrock(2), -5, 5, 4, 10, 1, 1, 1e-8, 1, trajectory=(-1, 4))
+    with pytest.raises(ValueError):
+        z.run_batch([pr])
 
 
 def test_dijet_generator():

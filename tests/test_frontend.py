@@ -34,8 +34,10 @@ def test_frontend_host_only():
 
 
 @pytest.fixture(scope="module")
-def gpu_cases():
-    p = subprocess.run([_exe()], capture_output=True, text=True, timeout=300)
+def gpu_cases(tmp_path_factory):
+    traj = str(tmp_path_factory.mktemp("frontend") / "traj.tsv")
+    env = dict(os.environ, ZEUS_FRONTEND_TRAJ=traj)
+    p = subprocess.run([_exe()], capture_output=True, text=True, timeout=300, env=env)
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
     cases = {}
     for line in p.stdout.splitlines():
@@ -43,6 +45,7 @@ def gpu_cases():
             d = json.loads(line)
             cases[d["case"]] = d
     assert cases["checks"]["failures"] == 0, p.stderr
+    cases["traj_path"] = traj
     return cases
 
 
@@ -126,3 +129,25 @@ def test_frontend_early_stop(gpu_cases):
     assert got["claimed"] + got["stopped"] + got["surrendered"] == 512
     assert got["status"] == 1
     assert np.abs(np.array(got["x"]) - 1.0).max() <= 1e-4
+
+
+@pytest.mark.gpu
+def test_frontend_trajectory_file(orc, gpu_cases):
+    """zeus::Zeus(..., trajectory_file) from a user functor writes the TSV of
+    the per-thread run, bounded to 50 steps, equal to the oracle's recording."""
+    o = orc.run(orc.ROSENBROCK, 2, -5, 5, 32, 300, 3, 33, 1e-8, 42, grad=orc.GRAD_DUAL,
+                trajectory=(50, 32), trajectory_alpha=False)
+    got = gpu_cases["rosenbrock2_thread_traj"]
+    assert got["idx"] == o.best["idx"] and got["fval"] == o.best["fval"]
+    with open(gpu_cases["traj_path"]) as fh:
+        head = fh.readline().rstrip("\n").split("\t")
+        rows = [line.rstrip("\n").split("\t") for line in fh]
+    assert head == ["traj", "step", "fval", "grad", "status", "alpha", "x0", "x1"]
+    assert len(rows) == 32 * 50
+    t = o.trajectory
+    for r in rows:
+        i, it = int(r[0]), int(r[1])
+        for col, arr in ((2, t["fval"][it, i]), (3, t["grad"][it, i]),
+                         (6, t["coords"][it, 0, i]), (7, t["coords"][it, 1, i])):
+            assert (r[col] == "NaN") if np.isnan(arr) else float(r[col]) == arr
+        assert int(r[4]) == t["status"][it, i] and float(r[5]) == 0.0

@@ -11,7 +11,7 @@ import os
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libzeus_amd.so")
 
-OK, E_ALLOC, E_LAUNCH, E_ARG, E_UNSUPPORTED, E_NODEVICE = 0, 3, 4, -10, -11, -12
+OK, E_ALLOC, E_LAUNCH, E_ARG, E_UNSUPPORTED, E_NODEVICE, E_IO = 0, 3, 4, -10, -11, -12, -13
 UNSET, SURRENDERED, CONVERGED, STOPPED, MALLOC, KERNEL, NONFINITE = -1, 0, 1, 2, 3, 4, 5
 OBJ_ROSENBROCK, OBJ_RASTRIGIN, OBJ_ACKLEY, OBJ_GAUSSIAN, OBJ_LOGLIK, OBJ_MLP, OBJ_SPHERE = (
     1, 2, 3, 4, 5, 6, 7)
@@ -22,13 +22,19 @@ _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
 
 
+class Trajectory(C.Structure):
+    _fields_ = [("coords", C.c_void_p), ("fval", C.c_void_p), ("grad", C.c_void_p),
+                ("status", C.c_void_p), ("alpha", C.c_void_p), ("steps", C.c_int32),
+                ("starts", C.c_int32)]
+
+
 class Problem(C.Structure):
     _fields_ = [("objective", C.c_int32), ("dim", C.c_int32), ("lower", C.c_double),
                 ("upper", C.c_double), ("n_starts", C.c_int64), ("max_iter", C.c_int32),
                 ("pso_iter", C.c_int32), ("required", C.c_int32), ("seed", C.c_uint32),
                 ("stream", C.c_uint32), ("reserved", C.c_int32), ("tol", C.c_double),
                 ("params", C.c_void_p), ("n_params", C.c_int64), ("x0", C.c_void_p),
-                ("name", C.c_char_p)]
+                ("name", C.c_char_p), ("trajectory", C.POINTER(Trajectory))]
 
 
 class Options(C.Structure):
@@ -55,6 +61,8 @@ SIGNATURES = {
     "zeus_run_batch": (C.c_int32, [C.POINTER(Problem), C.c_int32, C.POINTER(Options),
                                    C.POINTER(Result), _dp, C.POINTER(StartOutputs),
                                    C.c_void_p]),
+    "zeus_write_trajectory": (C.c_int32, [C.c_char_p, _dp, _dp, _dp, C.c_void_p, _dp,
+                                          C.c_int32, C.c_int32, C.c_int32]),
     "zeus_pso_run": (C.c_int32, [C.POINTER(Problem), C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p]),
     "zeus_eval": (C.c_int32, [C.POINTER(Problem), C.c_void_p, C.c_int64, C.c_void_p,

@@ -12,6 +12,7 @@ from __future__ import annotations
 import ctypes as C
 import enum
 import math
+import warnings
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -24,6 +25,7 @@ __all__ = [
     "Rastrigin", "Ackley", "Sphere", "Gaussian", "LogLikelihood", "Mlp", "Zeus",
     "Problem", "run_batch", "pso", "evaluate", "argmin", "philox_uniform", "philox_fill",
     "make_dijet", "make_mlp_data", "make_spd", "tuned_config", "set_fault_hooks",
+    "write_trajectory",
 ]
 
 
@@ -60,6 +62,7 @@ class Result:
     sum_ls_evals: int = 0
     sum_grads: int = 0
     per_start: dict | None = None
+    trajectory: dict | None = None
 
 
 class DeviceArray:
@@ -204,6 +207,38 @@ class Problem:
     fun_name: str | None = None
     stream: int = 0
     x0: np.ndarray | None = None
+    # (steps, starts): record the first `steps` iterations of the first
+    # `starts` starts (zeus_trajectory); alpha: also the line-search steps
+    trajectory: tuple | None = None
+    trajectory_alpha: bool = True
+
+
+class _TrajBuffers:
+    """Device buffers of one problem's trajectory and their C descriptor."""
+
+    def __init__(self, steps: int, starts: int, dim: int, alpha: bool):
+        if steps < 0 or starts < 0:
+            raise ValueError("trajectory steps and starts must be >= 0")
+        self.steps, self.starts, self.dim = int(steps), int(starts), int(dim)
+        n = self.steps * self.starts
+        self.coords = DeviceArray(nbytes=8 * n * dim)
+        self.fval = DeviceArray(nbytes=8 * n)
+        self.grad = DeviceArray(nbytes=8 * n)
+        self.status = DeviceArray(nbytes=n, dtype=np.int8)
+        self.alpha = DeviceArray(nbytes=8 * n) if alpha else None
+        self.c = capi.Trajectory(self.coords.addr or None, self.fval.addr or None,
+                                 self.grad.addr or None, self.status.addr or None,
+                                 (self.alpha.addr or None) if alpha else None,
+                                 self.steps, self.starts)
+
+    def to_host(self) -> dict:
+        T, S, D = self.steps, self.starts, self.dim
+        return dict(coords=self.coords.to_host().reshape(T, D, S),
+                    fval=self.fval.to_host().reshape(T, S),
+                    grad=self.grad.to_host().reshape(T, S),
+                    status=self.status.to_host().reshape(T, S),
+                    alpha=(self.alpha.to_host().reshape(T, S) if self.alpha is not None
+                           else np.zeros((T, S))))
 
 
 def _options(gradient="adjoint", form=None, parallel=True, group_threads=0, nzerosteps=0,
@@ -223,6 +258,7 @@ def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group
     """Many independent problems in one launch sequence (zeus_run_batch)."""
     n = len(problems)
     keep = []
+    trajs = {}
     P = (capi.Problem * max(n, 1))()
     for i, pr in enumerate(problems):
         addr, nparam = pr.f.device_params()
@@ -234,10 +270,18 @@ def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group
             x0addr = dx.addr
         name = (pr.fun_name if pr.fun_name is not None else pr.f.name).encode()
         keep.append(name)
+        tptr = None
+        if pr.trajectory is not None:
+            tb = _TrajBuffers(pr.trajectory[0], pr.trajectory[1], pr.f.dim,
+                              pr.trajectory_alpha)
+            trajs[i] = tb
+            tptr = C.pointer(tb.c)
+            keep.append(tptr)
         P[i] = capi.Problem(pr.f.kind, pr.f.dim, float(pr.lower), float(pr.upper), int(pr.N),
                             int(pr.MAX_ITER), int(pr.PSO_ITER), int(pr.requiredConverged),
                             int(pr.seed) & 0xFFFFFFFF, int(pr.stream) & 0xFFFFFFFF, 0,
-                            float(pr.tolerance), addr or None, nparam, x0addr or None, name)
+                            float(pr.tolerance), addr or None, nparam, x0addr or None, name,
+                            tptr)
     opt = _options(gradient, form, True, group_threads, 0, use_graph)
     R = (capi.Result * max(n, 1))()
     dims = sum(pr.f.dim for pr in problems)
@@ -270,6 +314,8 @@ def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group
                      sum_grads=r.sum_grads)
         if rc in (capi.E_ALLOC, capi.E_LAUNCH):
             res.status = capi.MALLOC if rc == capi.E_ALLOC else capi.KERNEL
+        if i in trajs and rc == capi.OK:
+            res.trajectory = trajs[i].to_host()
         if arrays is not None:
             res.per_start = {k: (v[os_:os_ + pr.N] if k != "x" else
                                  v[oxx:oxx + pr.N * pr.f.dim].reshape(pr.N, pr.f.dim))
@@ -285,26 +331,59 @@ def Zeus(f: Objective, lower: float, upper: float, N: int, MAX_ITER: int, PSO_IT
          requiredConverged: int, fun_name: str, tolerance: float, seed: int, run: int,
          parallel: bool = True, prng: PRNGType = PRNGType.XORWOW, trajectory_file: str = "",
          nzerosteps: int = 0, *, gradient: str = "adjoint", form=None, group_threads: int = 0,
-         x0=None, stream: int = 0, per_start: bool = False, use_graph: bool = False) -> Result:
+         x0=None, stream: int = 0, per_start: bool = False, use_graph: bool = False,
+         trajectory_steps: int | None = None, trajectory_starts: int | None = None) -> Result:
     """Multistart PSO -> BFGS; returns the best start's Result.
 
     Argument order and meaning follow zeus::Zeus (include/zeus.cuh:341-358).
     ``parallel`` selects the per-workgroup form (True) or the per-thread form
     (False) unless ``form`` overrides it; ``run`` only labels the call;
     ``prng`` is accepted for API parity (every value draws from Philox).
+    ``trajectory_file`` (".tsv") records every start's iterations and writes
+    them as reference zeus.cuh:189-239 does, the line-search steps only for
+    ``parallel`` (the reference's sequential form writes zeros); the arrays
+    are also returned in ``Result.trajectory``.  ``trajectory_steps`` /
+    ``trajectory_starts`` bound the recording (default MAX_ITER and N).
     """
-    if trajectory_file:
-        raise NotImplementedError("trajectory recording is not part of this build")
     if N < 0:
         raise ValueError("N must be >= 0")
     if N == 0:
         return Result(coordinates=np.zeros(f.dim))  # zeus.cuh:58-60
+    traj = None
+    if trajectory_file:
+        steps = MAX_ITER if trajectory_steps is None else min(int(trajectory_steps), MAX_ITER)
+        starts = N if trajectory_starts is None else min(int(trajectory_starts), N)
+        traj = (steps, starts)
     pr = Problem(f, lower, upper, N, MAX_ITER, PSO_ITER, requiredConverged, tolerance, seed,
-                 fun_name, stream, x0)
+                 fun_name, stream, x0, traj, bool(parallel))
     if form is None:
         form = "group" if parallel else "thread"
-    return run_batch([pr], gradient=gradient, form=form, group_threads=group_threads,
-                     per_start=per_start, use_graph=use_graph)[0]
+    res = run_batch([pr], gradient=gradient, form=form, group_threads=group_threads,
+                    per_start=per_start, use_graph=use_graph)[0]
+    if traj is not None and res.trajectory is not None:
+        # like the reference (zeus.cuh:229-238), a file that cannot be written
+        # is reported and the result still returned
+        try:
+            write_trajectory(trajectory_file, res.trajectory)
+        except capi.ZeusError as e:
+            warnings.warn(f"trajectory not written to {trajectory_file}: {e}")
+    return res
+
+
+def write_trajectory(filename: str, traj: dict) -> None:
+    """Write Result.trajectory arrays (zeus_write_trajectory; reference
+    util::writeTrajectoryData, src/utils.cu:126-286)."""
+    coords = np.ascontiguousarray(traj["coords"], np.float64)
+    steps, dim, starts = coords.shape
+    fval = np.ascontiguousarray(traj["fval"], np.float64)
+    grad = np.ascontiguousarray(traj["grad"], np.float64)
+    status = np.ascontiguousarray(traj["status"], np.int8)
+    alpha = np.ascontiguousarray(traj["alpha"], np.float64)
+    check(lib().zeus_write_trajectory(filename.encode(), coords.ctypes.data_as(capi._dp),
+                                      fval.ctypes.data_as(capi._dp),
+                                      grad.ctypes.data_as(capi._dp), status.ctypes.data,
+                                      alpha.ctypes.data_as(capi._dp), starts, steps, dim),
+          "zeus_write_trajectory")
 
 
 # --- phases and utilities -------------------------------------------------------
@@ -313,7 +392,7 @@ def pso(f: Objective, lower, upper, N, PSO_ITER, seed, stream=0):
     addr, nparam = f.device_params()
     P = capi.Problem(f.kind, f.dim, float(lower), float(upper), int(N), 0, int(PSO_ITER), 0,
                      int(seed) & 0xFFFFFFFF, int(stream) & 0xFFFFFFFF, 0, 0.0, addr or None,
-                     nparam, None, None)
+                     nparam, None, None, None)
     px = DeviceArray(nbytes=8 * N * f.dim)
     pv = DeviceArray(nbytes=8 * N)
     gx = DeviceArray(nbytes=8 * f.dim)
@@ -328,7 +407,7 @@ def evaluate(f: Objective, X, gradient: str | None = None):
     n = X.shape[0]
     addr, nparam = f.device_params()
     P = capi.Problem(f.kind, f.dim, 0.0, 0.0, 0, 0, 0, 0, 0, 0, 0, 0.0, addr or None, nparam,
-                     None, None)
+                     None, None, None)
     dx = DeviceArray(X)
     df = DeviceArray(nbytes=8 * n)
     dg = DeviceArray(nbytes=8 * n * f.dim) if gradient else None

@@ -11,6 +11,23 @@
 
 namespace zeus::dev {
 
+// Trajectory buffers of one problem (what reference include/zeus.cuh:62-103
+// allocates and bfgs_sequential.cuh:100-109 fills): coordinates in
+// [step][dim][start] order, fval (best value so far), grad (its gradient
+// norm), status and alpha (line-search step) in [step][start] order.  Only
+// the first `starts` starts and the first `steps` iterations are kept, so
+// the buffers can be bounded (the reference keeps all N x MAX_ITER).
+// fval == nullptr: nothing is recorded; alpha may be null on its own.
+struct TrajOut {
+  double* coords;
+  double* fval;
+  double* grad;
+  int8_t* status;
+  double* alpha;
+  int32_t steps;
+  int32_t starts;
+};
+
 struct ProblemDesc {
   int32_t objective;
   int32_t dim;
@@ -32,6 +49,7 @@ struct ProblemDesc {
   const double* x0;     // optional device start points (n_starts x dim)
   const double* aux;    // library-prepared objective data (Gaussian: Cs)
   const double* aux_frag; // Gaussian: Cs in MFMA A-operand order (cs_product.hpp)
+  TrajOut traj;           // trajectory recording (traj.fval == nullptr: off)
 };
 
 // Per-problem control block; zeroed before each call.
@@ -110,6 +128,28 @@ ZEUS_D ZEUS_FI int
 fetch_add(int32_t* p, int v)
 {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Trajectory slot of problem-local start sl at iteration iter, or -1 when
+// that step is not kept.
+ZEUS_HD ZEUS_FI int64_t
+traj_slot(const TrajOut& t, int64_t sl, int iter)
+{
+  if (t.fval == nullptr || sl >= t.starts || iter < 0 || iter >= t.steps)
+    return -1;
+  return int64_t(iter) * t.starts + sl;
+}
+
+// The status of a finished start, at the iteration the reference writes it
+// (bfgs_common.cuh:108-238): the breaking iteration for stopped (2),
+// non-finite (5) and converged (1); MAX_ITER - 1 for surrendered (0).
+ZEUS_D ZEUS_FI void
+traj_final_status(const TrajOut& t, int64_t sl, int status, int iter, int max_iter)
+{
+  const int at = status == kStatusSurrendered ? max_iter - 1 : iter;
+  const int64_t o = traj_slot(t, sl, at);
+  if (o >= 0)
+    t.status[o] = int8_t(status);
 }
 
 // Count a converged start and raise the stop flag when it is the

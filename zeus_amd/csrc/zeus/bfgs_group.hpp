@@ -504,6 +504,21 @@ group_bfgs(const F& f, GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval&
   bool done = false;
 
   for (; iter < P.max_iter; ++iter) {
+    // trajectory step (bfgs_parallel.cuh:155-166); uniform per workgroup
+    const int64_t to = traj_slot(P.traj, sl, iter);
+    if (to >= 0) {
+      double gg = 0.0;
+      for (int i = tid; i < D; i += NT)
+        gg = ::fma(L.g[i], L.g[i], gg);
+      const double nrm = ::sqrt(blk.sum1(gg));
+      for (int d = tid; d < D; d += NT)
+        P.traj.coords[(int64_t(iter) * D + d) * P.traj.starts + sl] = L.x[d];
+      if (tid == 0) {
+        P.traj.fval[to] = bestVal;
+        P.traj.grad[to] = nrm;
+        P.traj.status[to] = int8_t(kStatusUnset);
+      }
+    }
     if (tid == 0)
       L.flag = load_flag(&ctl->stop_flag);
     __syncthreads();
@@ -536,6 +551,8 @@ group_bfgs(const F& f, GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval&
       }
       alpha *= 0.5;
     }
+    if (tid == 0 && to >= 0 && P.traj.alpha) // bfgs_parallel.cuh:182-184
+      P.traj.alpha[to] = alpha;
     double fnew = f1;
     if (!accepted) {
       for (int j = tid; j < D; j += NT)
@@ -642,6 +659,7 @@ group_bfgs(const F& f, GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval&
     e2 = blk.sum1(t);
   }
   if (tid == 0) {
+    traj_final_status(P.traj, sl, status, iter, P.max_iter);
     out.status[s] = status;
     out.iter[s] = iter;
     out.fval[s] = fval;

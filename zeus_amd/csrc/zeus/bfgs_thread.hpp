@@ -88,6 +88,21 @@ thread_bfgs(const F& f, const ProblemDesc& P, Control* ctl, int64_t s,
   int iter = 0;
   bool done = false;
   for (; iter < P.max_iter; ++iter) {
+    // trajectory step: the point, its value and gradient norm, status -1
+    // (bfgs_sequential.cuh:100-109)
+    const int64_t to = traj_slot(P.traj, sl, iter);
+    if (to >= 0) {
+      double gg = 0.0;
+#pragma unroll(kU)
+      for (int i = 0; i < D; ++i)
+        gg = ::fma(g[i], g[i], gg);
+#pragma unroll(kU)
+      for (int d = 0; d < D; ++d)
+        P.traj.coords[(int64_t(iter) * D + d) * P.traj.starts + sl] = x[d];
+      P.traj.fval[to] = bestVal;
+      P.traj.grad[to] = ::sqrt(gg);
+      P.traj.status[to] = int8_t(kStatusUnset);
+    }
     // stop flag raised by another start (bfgs_common.cuh:115-141)
     if (load_flag(&ctl->stop_flag) != 0) {
       status = kStatusStopped;
@@ -128,6 +143,8 @@ thread_bfgs(const F& f, const ProblemDesc& P, Control* ctl, int64_t s,
       }
       alpha *= 0.5;
     }
+    if (to >= 0 && P.traj.alpha)
+      P.traj.alpha[to] = alpha;
 #pragma unroll(kU)
     for (int i = 0; i < D; ++i) {
       xt[i] = ::fma(alpha, p[i], x[i]);
@@ -211,6 +228,7 @@ thread_bfgs(const F& f, const ProblemDesc& P, Control* ctl, int64_t s,
       gg = ::fma(g[i], g[i], gg);
     gnorm = ::sqrt(gg);
   }
+  traj_final_status(P.traj, sl, status, iter, P.max_iter);
   out.status[s] = status;
   out.iter[s] = iter;
   out.fval[s] = fval;

@@ -34,6 +34,26 @@ zero_words_kernel(uint32_t* p, int n)
     p[i] = 0u;
 }
 
+// Trajectory buffers before a run (reference zeus.cuh:77-103): NaN
+// coordinates, values and gradient norms, status -1, alpha 0.
+static __global__ void
+traj_fill_kernel(TrajOut t, int D)
+{
+  const int64_t n = int64_t(t.steps) * t.starts, nc = n * D;
+  const double nan = __builtin_nan("");
+  const int64_t step = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nc; i += step) {
+    t.coords[i] = nan;
+    if (i < n) {
+      t.fval[i] = nan;
+      t.grad[i] = nan;
+      t.status[i] = int8_t(kStatusUnset);
+      if (t.alpha)
+        t.alpha[i] = 0.0;
+    }
+  }
+}
+
 // Cs = (C + C^T) / 2 for a Gaussian problem (same quadratic form), row
 // major, and the same matrix in MFMA A-operand order: fragment (mb, kb) is
 // 64 consecutive doubles, lane l holding Cs[16 mb + l % 16][4 kb + l / 16]

@@ -3,7 +3,9 @@
 //   auto r = zeus::Zeus(f, lower, upper, N, MAX_ITER, PSO_ITER,
 //                       requiredConverged, fun_name, tolerance, seed, run,
 //                       parallel = true, prng = PRNGType::XORWOW,
-//                       trajectory_file = {}, nzerosteps = 0);
+//                       trajectory_file = {}, nzerosteps = 0,
+//                       gradient = Gradient::Adjoint, trajectory_steps = -1,
+//                       trajectory_starts = -1);
 //
 // Same signature, defaults, Result<D> fields and status codes as the
 // reference (include/zeus.cuh:341-375).  F is any reference-style callable
@@ -39,6 +41,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -82,10 +85,20 @@ min_kind_of(std::string_view name)
   return 0;
 }
 
+// Trajectory request of one call: file, bounds, and whether the line-search
+// steps are kept (the reference records them only in its parallel form).
+struct TrajSpec {
+  std::string file;
+  int steps = 0;
+  int starts = 0;
+  bool alpha = false;
+};
+
 template <class F, int D, Gradient G>
 Result<D>
 run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO_ITER,
-    int required, std::string_view fun_name, double tol, int seed, bool parallel)
+    int required, std::string_view fun_name, double tol, int seed, bool parallel,
+    const TrajSpec* tr = nullptr)
 {
   Result<D> res;
   if (N == 0)
@@ -127,6 +140,16 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     P.stream = 0;
     P.tol = tol;
     P.x0 = pso ? pbx.data() : nullptr;
+    const int64_t t_steps = tr ? tr->steps : 0, t_starts = tr ? tr->starts : 0;
+    const int64_t t_n = t_steps * t_starts;
+    HipBuffer<double> t_coords(t_n * D), t_fval(t_n), t_grad(t_n),
+      t_alpha(tr && tr->alpha ? t_n : 0);
+    HipBuffer<int8_t> t_status(t_n);
+    if (t_n > 0) {
+      P.traj = TrajOut{t_coords.data(), t_fval.data(), t_grad.data(), t_status.data(),
+                       tr->alpha ? t_alpha.data() : nullptr, int32_t(t_steps),
+                       int32_t(t_starts)};
+    }
     HipBuffer<ProblemDesc> dP(1);
     abi_check(zeus_memcpy_h2d(dP.data(), &P, sizeof(P)), "upload");
     HipBuffer<Control> ctl(1);
@@ -177,6 +200,11 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
       }
     }
     hip_check(hipGetLastError(), "PSO launch");
+    if (t_n > 0) {
+      traj_fill_kernel<<<int(std::min<int64_t>((t_n * D + 255) / 256, 4096)), 256, 0, s>>>(
+        P.traj, D);
+      hip_check(hipGetLastError(), "trajectory fill");
+    }
     hip_check(hipEventRecord(e0, s), "event");
     ClassArgs a{dP.data(), 1, D, n, ctl.data(), nullptr, out, hslab.data(), stride};
     if (parallel)
@@ -206,6 +234,22 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     res.c.stopped = k.stopped;
     res.c.surrendered = k.surrendered;
     res.ms_opt = ms;
+    if (t_n > 0) { // zeus.cuh:189-239
+      std::vector<double> hc(t_n * D), hf(t_n), hg(t_n), ha(t_n, 0.0);
+      std::vector<int8_t> hs(t_n);
+      abi_check(zeus_memcpy_d2h(hc.data(), t_coords.data(), 8 * hc.size()), "trajectory copy");
+      abi_check(zeus_memcpy_d2h(hf.data(), t_fval.data(), 8 * hf.size()), "trajectory copy");
+      abi_check(zeus_memcpy_d2h(hg.data(), t_grad.data(), 8 * hg.size()), "trajectory copy");
+      abi_check(zeus_memcpy_d2h(hs.data(), t_status.data(), hs.size()), "trajectory copy");
+      if (tr->alpha)
+        abi_check(zeus_memcpy_d2h(ha.data(), t_alpha.data(), 8 * ha.size()), "trajectory copy");
+      const int rc = zeus_write_trajectory(tr->file.c_str(), hc.data(), hf.data(), hg.data(),
+                                           hs.data(), ha.data(), int32_t(t_starts),
+                                           int32_t(t_steps), D);
+      if (rc != ZEUS_OK) // reported, the result still returned (zeus.cuh:229-238)
+        std::fprintf(stderr, "zeus: trajectory not written to %s: %s\n", tr->file.c_str(),
+                     zeus_status_string(rc));
+    }
   } catch (const HipError& e) {
     res = Result<D>{};
     res.status = (e.code() == hipErrorOutOfMemory || e.code() == hipErrorMemoryAllocation)
@@ -224,22 +268,32 @@ Zeus(Function const& f, double lower, double upper, std::size_t N, int MAX_ITER,
      int requiredConverged, std::string_view fun_name, double tolerance, int seed, int run,
      bool parallel = true, PRNGType prng = PRNGType::XORWOW,
      std::string_view trajectory_file = {}, int nzerosteps = 0,
-     Gradient gradient = Gradient::Adjoint) -> Result<ZEUS_DIM>
+     Gradient gradient = Gradient::Adjoint, int trajectory_steps = -1,
+     int trajectory_starts = -1) -> Result<ZEUS_DIM>
 {
   (void)run;        // labels the call only
   (void)prng;       // every PRNGType draws from the counter-based Philox stream
   (void)nzerosteps; // inert: the line search never returns a zero step
-  if (!trajectory_file.empty())
-    throw std::invalid_argument("zeus::Zeus: trajectory recording is not part of this build");
   constexpr int D = int(ZEUS_DIM);
+  // trajectory_file: every start's iterations, written as .tsv (bounded by
+  // trajectory_steps / trajectory_starts; -1 = MAX_ITER / N)
+  detail::TrajSpec spec;
+  const detail::TrajSpec* tr = nullptr;
+  if (!trajectory_file.empty() && N > 0 && MAX_ITER > 0) {
+    spec.file = std::string(trajectory_file);
+    spec.steps = trajectory_steps < 0 ? MAX_ITER : std::min(trajectory_steps, MAX_ITER);
+    spec.starts = trajectory_starts < 0 ? int(N) : int(std::min<std::size_t>(trajectory_starts, N));
+    spec.alpha = parallel;
+    tr = &spec;
+  }
   if constexpr (HasAdjoint<Function>)
     if (gradient == Gradient::Adjoint)
       return detail::run<Function, D, Gradient::Adjoint>(f, lower, upper, N, MAX_ITER, PSO_ITER,
                                                          requiredConverged, fun_name, tolerance,
-                                                         seed, parallel);
+                                                         seed, parallel, tr);
   return detail::run<Function, D, Gradient::Dual>(f, lower, upper, N, MAX_ITER, PSO_ITER,
                                                   requiredConverged, fun_name, tolerance, seed,
-                                                  parallel);
+                                                  parallel, tr);
 }
 
 } // namespace zeus

@@ -247,9 +247,33 @@ validate(const zeus_problem& p)
       break;
     default: return ZEUS_E_ARG;
   }
+  if (const zeus_trajectory* t = p.trajectory) {
+    if (t->steps < 0 || t->starts < 0)
+      return ZEUS_E_ARG;
+    if (int64_t(t->steps) * t->starts > 0 && (!t->coords || !t->fval || !t->grad || !t->status))
+      return ZEUS_E_ARG;
+  }
   if (!rt::find_class(p.objective, p.dim))
     return ZEUS_E_UNSUPPORTED;
   return ZEUS_OK;
+}
+
+// Device view of a problem's trajectory request (fval == nullptr: off).
+dev::TrajOut
+traj_of(const zeus_problem& p)
+{
+  dev::TrajOut t{};
+  const zeus_trajectory* u = p.trajectory;
+  if (!u || int64_t(u->steps) * u->starts == 0)
+    return t;
+  t.coords = u->coords;
+  t.fval = u->fval;
+  t.grad = u->grad;
+  t.status = u->status;
+  t.alpha = u->alpha;
+  t.steps = u->steps;
+  t.starts = u->starts;
+  return t;
 }
 
 void
@@ -319,6 +343,7 @@ struct ClassPlan {
   double* aux = nullptr; // Gaussian Cs, one D x D block per problem
   double* h_slab = nullptr;
   int ctl0 = 0;          // first control block of the class
+  bool traced = false;   // some problem records a trajectory
   // host
   std::vector<ProblemDesc> desc;
   std::vector<int32_t> block_problem;
@@ -557,6 +582,8 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       d.params = p.params;
       d.n_params = p.n_params;
       d.x0 = p.x0;
+      d.traj = traj_of(p);
+      c.traced = c.traced || d.traj.fval != nullptr;
       c.desc.push_back(d);
       const int nb = int((p.n_starts + c.ops->pso_ppb - 1) / c.ops->pso_ppb);
       for (int b = 0; b < nb; ++b)
@@ -569,6 +596,19 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     c.pso_blocks = b0;
     c.key = TuneKey{c.ops->objective, c.ops->dim, opt.gradient, size_bucket(c.total)};
     c.cands = candidates(c.ops, opt);
+    if (c.traced) {
+      // recording runs one shape, not the tile form, and is not timed
+      // against others (each trial would rewrite the trajectory)
+      if (opt.form == ZEUS_FORM_TILE)
+        return ZEUS_E_ARG;
+      std::vector<TuneVal> keep;
+      for (const auto& v : c.cands)
+        if (v.form != ZEUS_FORM_TILE)
+          keep.push_back(v);
+      if (keep.empty())
+        keep.push_back({ZEUS_FORM_GROUP, c.ops->group_nt[c.ops->n_group - 1]});
+      c.cands = keep;
+    }
     if (c.cands.size() > 1) {
       auto it = g_ds.tuned.find(c.key);
       if (it != g_ds.tuned.end())
@@ -576,6 +616,8 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
           if (v.form == it->second.form && v.threads == it->second.threads)
             c.cands = {v};
     }
+    if (c.traced && c.cands.size() > 1)
+      c.cands.resize(1);
     c.cfg = c.cands.front();
     c.slab = 0;
     for (const auto& v : c.cands)
@@ -679,6 +721,16 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     for (auto& c : classes)
       if ((err = launch_class(c, d_ctl, opt.gradient, st)) != hipSuccess)
         return err;
+    // trajectory buffers: NaN / -1 / 0 before the BFGS writes its steps
+    for (const auto& c : classes)
+      for (const auto& d : c.desc)
+        if (d.traj.fval) {
+          const int64_t n = int64_t(d.traj.steps) * d.traj.starts * d.dim;
+          const int g = int(std::min<int64_t>((n + 255) / 256, 4096));
+          dev::traj_fill_kernel<<<g, 256, 0, st>>>(d.traj, d.dim);
+        }
+    if ((err = hipGetLastError()) != hipSuccess)
+      return err;
     if (timed)
       (void)hipEventRecord(g_ds.ev[1], st);
     for (auto& c : classes) {
@@ -696,7 +748,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   };
   bool settled = true;
   for (const auto& c : classes)
-    settled = settled && c.cands.size() == 1;
+    settled = settled && c.cands.size() == 1 && !c.traced;
   bool graphed = false;
   if (opt.use_graph && settled) {
     std::vector<uint64_t> key = {uint64_t(reinterpret_cast<uintptr_t>(g_ds.arena)),
@@ -1093,6 +1145,7 @@ zeus_status_string(int32_t status)
     case ZEUS_E_ARG: return "invalid argument";
     case ZEUS_E_UNSUPPORTED: return "objective/dimension not instantiated";
     case ZEUS_E_NODEVICE: return "no gfx950 device";
+    case ZEUS_E_IO: return "file could not be written";
     default: return "unknown";
   }
 }
