@@ -131,8 +131,27 @@ typedef struct zeus_options {
                             shape and replay it as a hipGraph (once the
                             autotuner has settled the launch shapes; the
                             first call of a shape runs uncaptured)         */
-  int32_t flags;         /* reserved, 0                                    */
+  int32_t flags;         /* ZEUS_FLAG_* bits                               */
 } zeus_options;
+
+/* zeus_options.flags: time gradient evaluations and H updates per start in
+   the per-thread form and report them in zeus_result.ad / .bfgs (reference
+   bfgs_sequential.cuh:236-382; the reference's parallel form reports zeros,
+   and so do the workgroup forms here). */
+#define ZEUS_FLAG_METRICS 1
+
+/* Timing summary of one kind of work over a problem's starts (reference
+   Metrics, include/util.hpp:16-25; computed as report_metrics_and_cleanup,
+   bfgs_sequential.cuh:240-382, from a constant-rate device clock). */
+typedef struct zeus_metrics {
+  double ms_per_call;           /* mean time per call                      */
+  double calls_per_thread_mean; /* mean calls per start                    */
+  double fraction_of_kernel;    /* mean time per start / kernel time       */
+  double block95;               /* 95th percentile over 256-start blocks of
+                                   (summed block time / kernel time)       */
+  double serialized;            /* summed time over starts / kernel time   */
+  double fraction_of_thread;    /* mean over starts of time / start's total */
+} zeus_metrics;
 
 /* Per-problem outcome: the fields of zeus::Result<D> (include/util.hpp:36-54)
    except the coordinates, which go to the best_x argument. */
@@ -152,6 +171,8 @@ typedef struct zeus_result {
   int64_t sum_iter;     /* sum over starts of BFGS iterations              */
   int64_t sum_ls_evals; /* sum over starts of line-search evaluations      */
   int64_t sum_grads;    /* sum over starts of gradient evaluations         */
+  zeus_metrics ad;      /* gradient evaluations (ZEUS_FLAG_METRICS)        */
+  zeus_metrics bfgs;    /* H updates (ZEUS_FLAG_METRICS)                   */
 } zeus_result;
 
 /* Optional per-start host outputs, concatenated over problems in order;
@@ -223,6 +244,34 @@ zeus_status_t zeus_write_trajectory(const char* filename, const double* coords,
                                     const double* fval, const double* grad,
                                     const int8_t* status, const double* alpha,
                                     int32_t n, int32_t steps, int32_t dim);
+
+/* Metrics from per-start host arrays: ticks[n] and calls[n] of one kind of
+   work, total[n] the starts' whole times (NULL: fraction_of_thread = 0),
+   starts grouped in blocks of threads_per_block for block95, ms_kernel the
+   kernel time, tick_hz the clock rate. */
+zeus_status_t zeus_metrics_from_ticks(const uint64_t* ticks, const int32_t* calls,
+                                      const uint64_t* total, int64_t n,
+                                      int32_t threads_per_block, double ms_kernel,
+                                      double tick_hz, zeus_metrics* out);
+
+/* Distance of x from the known minimum implied by fun_name (reference
+   util::calculateEuclideanError, src/util.cpp:11-41): "rosenbrock" -> all
+   ones, "goldstein_price" -> (0, -1) (NaN unless dim == 2), otherwise the
+   origin. */
+double zeus_euclidean_error(const char* fun_name, const double* x, int32_t dim);
+
+/* Append one row for a call to the results table `path` (NULL:
+   "zeus_<dim>d_results.tsv" in the working directory), writing the header
+   first when the file is empty: reference util::appendResultsToTsv
+   (include/util.hpp:78-149), same columns and number formats.  best holds
+   the Result fields (and metrics), coords its dim coordinates. */
+zeus_status_t zeus_append_results_tsv(const char* path, int32_t dim, int64_t n,
+                                      const char* fun_name, double ms_init,
+                                      double ms_pso, double ms_opt, double ms_rand,
+                                      int32_t max_iter, int32_t pso_iter,
+                                      double error, int32_t run,
+                                      const zeus_result* best,
+                                      const double* coords);
 
 /* Name of a status code ("surrendered", "converged", ...). */
 const char* zeus_status_string(int32_t status);

@@ -35,9 +35,10 @@ def test_frontend_host_only():
 
 @pytest.fixture(scope="module")
 def gpu_cases(tmp_path_factory):
-    traj = str(tmp_path_factory.mktemp("frontend") / "traj.tsv")
+    wd = tmp_path_factory.mktemp("frontend")  # zeus::Zeus appends its results table here
+    traj = str(wd / "traj.tsv")
     env = dict(os.environ, ZEUS_FRONTEND_TRAJ=traj)
-    p = subprocess.run([_exe()], capture_output=True, text=True, timeout=300, env=env)
+    p = subprocess.run([_exe()], capture_output=True, text=True, timeout=300, env=env, cwd=wd)
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
     cases = {}
     for line in p.stdout.splitlines():
@@ -46,6 +47,7 @@ def gpu_cases(tmp_path_factory):
             cases[d["case"]] = d
     assert cases["checks"]["failures"] == 0, p.stderr
     cases["traj_path"] = traj
+    cases["workdir"] = str(wd)
     return cases
 
 

@@ -9,12 +9,14 @@ package is the Python host mirror of the reference's zeus::Zeus API.
 from .api import (Ackley, Convergence, DeviceArray, Gaussian, LogLikelihood, Mlp, Objective,
                   PRNGType, Problem, Rastrigin, Result, Rosenbrock, Sphere, Zeus, argmin,
                   evaluate, make_dijet, make_mlp_data, make_spd, philox_fill, philox_uniform,
-                  pso, run_batch, set_fault_hooks, tuned_config, write_trajectory)
+                  pso, run_batch, set_fault_hooks, tuned_config, write_trajectory,
+                  euclidean_error, append_results_tsv)
 from . import _capi as capi
 
 __all__ = [
     "Ackley", "Convergence", "DeviceArray", "Gaussian", "LogLikelihood", "Mlp", "Objective",
     "PRNGType", "Problem", "Rastrigin", "Result", "Rosenbrock", "Sphere", "Zeus", "argmin",
     "evaluate", "make_dijet", "make_mlp_data", "make_spd", "philox_fill", "philox_uniform",
-    "pso", "run_batch", "set_fault_hooks", "tuned_config", "write_trajectory", "capi",
+    "pso", "run_batch", "set_fault_hooks", "tuned_config", "write_trajectory",
+    "euclidean_error", "append_results_tsv", "capi",
 ]

@@ -17,6 +17,7 @@ OBJ_ROSENBROCK, OBJ_RASTRIGIN, OBJ_ACKLEY, OBJ_GAUSSIAN, OBJ_LOGLIK, OBJ_MLP, OB
     1, 2, 3, 4, 5, 6, 7)
 GRAD_ADJOINT, GRAD_DUAL = 0, 1
 FORM_AUTO, FORM_THREAD, FORM_GROUP, FORM_TILE = 0, 1, 2, 3
+FLAG_METRICS = 1
 
 _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
@@ -42,13 +43,20 @@ class Options(C.Structure):
                 ("nzerosteps", C.c_int32), ("use_graph", C.c_int32), ("flags", C.c_int32)]
 
 
+class Metrics(C.Structure):
+    _fields_ = [("ms_per_call", C.c_double), ("calls_per_thread_mean", C.c_double),
+                ("fraction_of_kernel", C.c_double), ("block95", C.c_double),
+                ("serialized", C.c_double), ("fraction_of_thread", C.c_double)]
+
+
 class Result(C.Structure):
     _fields_ = [("idx", C.c_int32), ("status", C.c_int32), ("iter", C.c_int32),
                 ("nonfinite", C.c_int32), ("fval", C.c_double), ("gradient_norm", C.c_double),
                 ("claimed", C.c_int32), ("actual", C.c_int32), ("surrendered", C.c_int32),
                 ("stopped", C.c_int32), ("ms_pso", C.c_double), ("ms_opt", C.c_double),
                 ("ms_total", C.c_double), ("sum_iter", C.c_int64),
-                ("sum_ls_evals", C.c_int64), ("sum_grads", C.c_int64)]
+                ("sum_ls_evals", C.c_int64), ("sum_grads", C.c_int64), ("ad", Metrics),
+                ("bfgs", Metrics)]
 
 
 class StartOutputs(C.Structure):
@@ -61,6 +69,14 @@ SIGNATURES = {
     "zeus_run_batch": (C.c_int32, [C.POINTER(Problem), C.c_int32, C.POINTER(Options),
                                    C.POINTER(Result), _dp, C.POINTER(StartOutputs),
                                    C.c_void_p]),
+    "zeus_metrics_from_ticks": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                            C.c_int32, C.c_double, C.c_double,
+                                            C.POINTER(Metrics)]),
+    "zeus_euclidean_error": (C.c_double, [C.c_char_p, _dp, C.c_int32]),
+    "zeus_append_results_tsv": (C.c_int32, [C.c_char_p, C.c_int32, C.c_int64, C.c_char_p,
+                                            C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                            C.POINTER(Result), _dp]),
     "zeus_write_trajectory": (C.c_int32, [C.c_char_p, _dp, _dp, _dp, C.c_void_p, _dp,
                                           C.c_int32, C.c_int32, C.c_int32]),
     "zeus_pso_run": (C.c_int32, [C.POINTER(Problem), C.c_void_p, C.c_void_p, C.c_void_p,

@@ -25,7 +25,7 @@ __all__ = [
     "Rastrigin", "Ackley", "Sphere", "Gaussian", "LogLikelihood", "Mlp", "Zeus",
     "Problem", "run_batch", "pso", "evaluate", "argmin", "philox_uniform", "philox_fill",
     "make_dijet", "make_mlp_data", "make_spd", "tuned_config", "set_fault_hooks",
-    "write_trajectory",
+    "write_trajectory", "euclidean_error", "append_results_tsv",
 ]
 
 
@@ -63,6 +63,10 @@ class Result:
     sum_grads: int = 0
     per_start: dict | None = None
     trajectory: dict | None = None
+    # timing of gradient evaluations (ad) and H updates (bfgs) per start,
+    # per-thread form only (reference Metrics; zeros otherwise)
+    ad: dict = field(default_factory=dict)
+    bfgs: dict = field(default_factory=dict)
 
 
 class DeviceArray:
@@ -242,7 +246,7 @@ class _TrajBuffers:
 
 
 def _options(gradient="adjoint", form=None, parallel=True, group_threads=0, nzerosteps=0,
-             use_graph=False):
+             use_graph=False, metrics=False):
     g = {"adjoint": capi.GRAD_ADJOINT, "dual": capi.GRAD_DUAL}[gradient]
     if form is None:
         form = capi.FORM_GROUP if parallel else capi.FORM_THREAD
@@ -250,11 +254,15 @@ def _options(gradient="adjoint", form=None, parallel=True, group_threads=0, nzer
         form = {"auto": capi.FORM_AUTO, "thread": capi.FORM_THREAD,
                 "group": capi.FORM_GROUP, "tile": capi.FORM_TILE}[form]
     return capi.Options(g, int(form), int(group_threads), int(nzerosteps),
-                        1 if use_graph else 0, 0)
+                        1 if use_graph else 0, capi.FLAG_METRICS if metrics else 0)
+
+
+def _metrics_dict(m) -> dict:
+    return {k: getattr(m, k) for k, _ in capi.Metrics._fields_}
 
 
 def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group_threads=0,
-              per_start=False, use_graph=False) -> list[Result]:
+              per_start=False, use_graph=False, metrics=False) -> list[Result]:
     """Many independent problems in one launch sequence (zeus_run_batch)."""
     n = len(problems)
     keep = []
@@ -282,7 +290,7 @@ def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group
                             int(pr.seed) & 0xFFFFFFFF, int(pr.stream) & 0xFFFFFFFF, 0,
                             float(pr.tolerance), addr or None, nparam, x0addr or None, name,
                             tptr)
-    opt = _options(gradient, form, True, group_threads, 0, use_graph)
+    opt = _options(gradient, form, True, group_threads, 0, use_graph, metrics)
     R = (capi.Result * max(n, 1))()
     dims = sum(pr.f.dim for pr in problems)
     bx = np.zeros(max(dims, 1))
@@ -311,7 +319,8 @@ def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group
                      c=Convergence(r.actual, r.claimed, r.surrendered, r.stopped),
                      ms_opt=r.ms_opt, ms_pso=r.ms_pso, ms_total=r.ms_total,
                      nonfinite=r.nonfinite, sum_iter=r.sum_iter, sum_ls_evals=r.sum_ls_evals,
-                     sum_grads=r.sum_grads)
+                     sum_grads=r.sum_grads, ad=_metrics_dict(r.ad),
+                     bfgs=_metrics_dict(r.bfgs))
         if rc in (capi.E_ALLOC, capi.E_LAUNCH):
             res.status = capi.MALLOC if rc == capi.E_ALLOC else capi.KERNEL
         if i in trajs and rc == capi.OK:
@@ -332,7 +341,8 @@ def Zeus(f: Objective, lower: float, upper: float, N: int, MAX_ITER: int, PSO_IT
          parallel: bool = True, prng: PRNGType = PRNGType.XORWOW, trajectory_file: str = "",
          nzerosteps: int = 0, *, gradient: str = "adjoint", form=None, group_threads: int = 0,
          x0=None, stream: int = 0, per_start: bool = False, use_graph: bool = False,
-         trajectory_steps: int | None = None, trajectory_starts: int | None = None) -> Result:
+         trajectory_steps: int | None = None, trajectory_starts: int | None = None,
+         results_tsv: bool | str = False) -> Result:
     """Multistart PSO -> BFGS; returns the best start's Result.
 
     Argument order and meaning follow zeus::Zeus (include/zeus.cuh:341-358).
@@ -344,6 +354,10 @@ def Zeus(f: Objective, lower: float, upper: float, N: int, MAX_ITER: int, PSO_IT
     ``parallel`` (the reference's sequential form writes zeros); the arrays
     are also returned in ``Result.trajectory``.  ``trajectory_steps`` /
     ``trajectory_starts`` bound the recording (default MAX_ITER and N).
+    With ``parallel=False`` the per-thread form times its gradients and
+    updates (``Result.ad`` / ``Result.bfgs``), as the reference's sequential
+    form does.  ``results_tsv`` (True or a path) appends the call's row to
+    zeus_<D>d_results.tsv (reference zeus.cuh:172-183).
     """
     if N < 0:
         raise ValueError("N must be >= 0")
@@ -359,7 +373,10 @@ def Zeus(f: Objective, lower: float, upper: float, N: int, MAX_ITER: int, PSO_IT
     if form is None:
         form = "group" if parallel else "thread"
     res = run_batch([pr], gradient=gradient, form=form, group_threads=group_threads,
-                    per_start=per_start, use_graph=use_graph)[0]
+                    per_start=per_start, use_graph=use_graph, metrics=not parallel)[0]
+    if results_tsv:
+        append_results_tsv(res, f.dim, N, fun_name, MAX_ITER, PSO_ITER, run,
+                           None if results_tsv is True else str(results_tsv))
     if traj is not None and res.trajectory is not None:
         # like the reference (zeus.cuh:229-238), a file that cannot be written
         # is reported and the result still returned
@@ -368,6 +385,35 @@ def Zeus(f: Objective, lower: float, upper: float, N: int, MAX_ITER: int, PSO_IT
         except capi.ZeusError as e:
             warnings.warn(f"trajectory not written to {trajectory_file}: {e}")
     return res
+
+
+def euclidean_error(fun_name: str, x) -> float:
+    """Distance to the known minimum implied by fun_name (reference
+    util::calculateEuclideanError, src/util.cpp:11-41)."""
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().zeus_euclidean_error(fun_name.encode(), x.ctypes.data_as(capi._dp), x.size)
+
+
+def append_results_tsv(res: Result, dim: int, N: int, fun_name: str, MAX_ITER: int,
+                       PSO_ITER: int, run: int, path: str | None = None) -> None:
+    """Append one call's row to the results table (zeus_append_results_tsv;
+    reference util::appendResultsToTsv, include/util.hpp:78-149)."""
+    r = capi.Result()
+    r.idx, r.status, r.iter = res.idx, res.status, res.iter
+    r.fval, r.gradient_norm = res.fval, res.gradientNorm
+    r.claimed, r.actual = res.c.claimed, res.c.actual
+    r.surrendered, r.stopped = res.c.surrendered, res.c.stopped
+    for name in ("ad", "bfgs"):
+        m = getattr(res, name) or {}
+        setattr(r, name, capi.Metrics(*[float(m.get(k, 0.0)) for k, _ in capi.Metrics._fields_]))
+    x = np.ascontiguousarray(res.coordinates, np.float64)
+    err = euclidean_error(fun_name, x)
+    check(lib().zeus_append_results_tsv(path.encode() if path else None, int(dim), int(N),
+                                        fun_name.encode(), 0.0, float(res.ms_pso),
+                                        float(res.ms_opt), 0.0, int(MAX_ITER), int(PSO_ITER),
+                                        err, int(run), C.byref(r),
+                                        x.ctypes.data_as(capi._dp)),
+          "zeus_append_results_tsv")
 
 
 def write_trajectory(filename: str, traj: dict) -> None:

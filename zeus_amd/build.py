@@ -5,8 +5,10 @@
 Every .hip file under zeus_amd/csrc is compiled with hipcc for gfx950 only
 into zeus_amd/_build/, then linked into zeus_amd/lib/libzeus_amd.so.
 Without --force, objects newer than every source and header are reused.
-The C++ front-end test programs under tests/cpp/*.hip are then compiled
-against the library into tests/cpp/bin/ (with an rpath to zeus_amd/lib).
+The example drivers under zeus_amd/drivers/*.hip (zeus_amd/bin/zeus_optimize,
+with optimize_<function> links to it) and the C++ front-end test programs
+under tests/cpp/*.hip (tests/cpp/bin/) are then compiled against the library
+(with an rpath to zeus_amd/lib).
 """
 from __future__ import annotations
 
@@ -24,6 +26,10 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libzeus_amd.so")
+DRIVERS = os.path.join(PKG, "drivers")
+BINDIR = os.path.join(PKG, "bin")
+# names the example-driver program answers to (examples/optimize_*.cu)
+DRIVER_FUNCTIONS = ("rosenbrock", "rastrigin", "ackley", "goldstein_price", "himmelblau")
 CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
 CPP_TEST_BIN = os.path.join(CPP_TESTS, "bin")
 
@@ -93,8 +99,45 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = True) ->
         os.replace(tmp, LIB)
         if verbose:
             print(f"[zeus_amd.build] linked {os.path.relpath(LIB, ROOT)}", flush=True)
+    build_programs(DRIVERS, BINDIR, force=force or bool(todo), verbose=verbose)
+    for fn in DRIVER_FUNCTIONS:
+        link = os.path.join(BINDIR, "optimize_" + fn)
+        if not os.path.islink(link):
+            if os.path.exists(link):
+                os.remove(link)
+            os.symlink("zeus_optimize", link)
     build_cpp_tests(force=force or bool(todo), verbose=verbose)
     return LIB
+
+
+def build_programs(src_dir: str, bin_dir: str, force: bool = False,
+                   verbose: bool = True) -> list[str]:
+    """Compile each src_dir/*.hip into bin_dir/<name> linked to the library
+    (the example drivers: optimize.hip -> zeus_optimize)."""
+    srcs = sorted(glob.glob(os.path.join(src_dir, "*.hip")))
+    if not srcs:
+        return []
+    os.makedirs(bin_dir, exist_ok=True)
+    hdrs = _headers() + glob.glob(os.path.join(src_dir, "*.hpp"))
+    newest = max([os.path.getmtime(h) for h in hdrs] + [os.path.getmtime(LIB)])
+    rpath = "-Wl,-rpath,$ORIGIN/" + os.path.relpath(LIBDIR, bin_dir)
+    out = []
+    for s in srcs:
+        name = os.path.basename(s)[:-4]
+        exe = os.path.join(bin_dir, "zeus_" + name if src_dir == DRIVERS else name)
+        out.append(exe)
+        if (not force and os.path.exists(exe) and os.path.getmtime(exe) >= os.path.getmtime(s)
+                and os.path.getmtime(exe) >= newest):
+            continue
+        cmd = [hipcc(), *HIPFLAGS, f"-I{src_dir}", s, "-o", exe, f"-L{LIBDIR}", "-lzeus_amd",
+               rpath]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            err = "\n".join(l for l in p.stderr.splitlines() if "packed-fp32-ops" not in l)
+            raise RuntimeError(f"compilation of {os.path.relpath(s, ROOT)} failed:\n{err}")
+        if verbose:
+            print(f"[zeus_amd.build] compiled {os.path.relpath(exe, ROOT)}", flush=True)
+    return out
 
 
 def build_cpp_tests(force: bool = False, verbose: bool = True) -> list[str]:

@@ -75,6 +75,15 @@ struct StartOut {
   int32_t* ls_evals; // objective evaluations made by line searches
   int32_t* grads;    // gradient evaluations
   int8_t* near;      // 1: converged within 0.5 (Euclidean) of the known minimum
+  // Optional per-start timing of the per-thread form (reference
+  // bfgs_sequential.cuh:46-233): constant-rate clock ticks spent in gradient
+  // evaluations and in H updates, their call counts, and the start's total.
+  // ad_ticks == nullptr: not collected.
+  uint64_t* ad_ticks;
+  int32_t* ad_calls;
+  uint64_t* bfgs_ticks;
+  int32_t* bfgs_calls;
+  uint64_t* total_ticks;
 };
 
 // Coordinate d of the known minimum selected by ProblemDesc::min_kind

@@ -58,6 +58,11 @@ thread_bfgs(const F& f, const ProblemDesc& P, Control* ctl, int64_t s,
 {
   constexpr int kU = D <= kThreadRegMaxD ? D : 1; // unroll factor
   const int64_t sl = s - P.start0; // start index within its problem
+  // timing (bfgs_sequential.cuh:50,86-97,149-173,222-232), when requested
+  const bool timed = out.ad_ticks != nullptr;
+  const uint64_t k_begin = timed ? uint64_t(wall_clock64()) : 0;
+  uint64_t ad_t = 0, bfgs_t = 0;
+  int ad_n = 0, bfgs_n = 0;
   double x[D], g[D], p[D], xt[D], gn[D], dx[D], dg[D];
 
 #pragma unroll(kU)
@@ -80,7 +85,14 @@ thread_bfgs(const F& f, const ProblemDesc& P, Control* ctl, int64_t s,
 
   const double f0 = lane_value<F, D>(f, x);
   double bestVal = f0;
-  lane_gradient<F, D, G>(f, x, g);
+  {
+    const uint64_t t0 = timed ? uint64_t(wall_clock64()) : 0;
+    lane_gradient<F, D, G>(f, x, g);
+    if (timed) {
+      ad_t += uint64_t(wall_clock64()) - t0;
+      ++ad_n;
+    }
+  }
   int grads = 1, ls_evals = 0;
 
   int status = kStatusUnset;
@@ -155,8 +167,16 @@ thread_bfgs(const F& f, const ProblemDesc& P, Control* ctl, int64_t s,
       fnew = lane_value<F, D>(f, xt);
       ++ls_evals;
     }
-    lane_gradient<F, D, G>(f, xt, gn);
+    {
+      const uint64_t t0 = timed ? uint64_t(wall_clock64()) : 0;
+      lane_gradient<F, D, G>(f, xt, gn);
+      if (timed) {
+        ad_t += uint64_t(wall_clock64()) - t0;
+        ++ad_n;
+      }
+    }
     ++grads;
+    const uint64_t b0 = timed ? uint64_t(wall_clock64()) : 0;
     double sTy = 0.0;
 #pragma unroll(kU)
     for (int i = 0; i < D; ++i) {
@@ -189,6 +209,10 @@ thread_bfgs(const F& f, const ProblemDesc& P, Control* ctl, int64_t s,
 #pragma unroll(kU)
         for (int j = i; j < D; ++j)
           H.at(i, j) = H.at(i, j) + (dx[i] * u[j] + u[i] * dx[j]);
+    }
+    if (timed) {
+      bfgs_t += uint64_t(wall_clock64()) - b0;
+      ++bfgs_n;
     }
     if (fnew < bestVal) {
       bestVal = fnew;
@@ -248,6 +272,13 @@ thread_bfgs(const F& f, const ProblemDesc& P, Control* ctl, int64_t s,
     near = ::sqrt(e2) < 0.5;
   }
   out.near[s] = near;
+  if (timed) {
+    out.ad_ticks[s] = ad_t;
+    out.ad_calls[s] = ad_n;
+    out.bfgs_ticks[s] = bfgs_t;
+    out.bfgs_calls[s] = bfgs_n;
+    out.total_ticks[s] = uint64_t(wall_clock64()) - k_begin;
+  }
 }
 
 template <class F, int D, Gradient G, class Prov>

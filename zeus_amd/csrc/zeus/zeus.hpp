@@ -32,6 +32,7 @@
 #include "matrix.hpp"
 #include "objectives.hpp"
 #include "pso.hpp"
+#include "report.hpp"
 #include "result.hpp"
 #include "summary.hpp"
 #include "traits.hpp"
@@ -50,7 +51,57 @@
 
 namespace zeus {
 
+// Every call appends its row to zeus_<D>d_results.tsv in the working
+// directory, as the reference does (include/zeus.cuh:172-183); a process
+// can switch that off here.
+inline bool&
+results_table_enabled()
+{
+  static bool on = true;
+  return on;
+}
+inline void
+set_results_table(bool on)
+{
+  results_table_enabled() = on;
+}
+
 namespace detail {
+
+// Metrics of one kind of work from per-start tick arrays, printed the way
+// reference report_metrics_and_cleanup does (bfgs_sequential.cuh:240-382).
+inline Metrics
+report_metrics(const char* label, const std::vector<uint64_t>& ticks,
+               const std::vector<int32_t>& calls, const std::vector<uint64_t>& total,
+               float ms_opt, double tick_hz)
+{
+  zeus_metrics m{};
+  const int64_t n = int64_t(ticks.size());
+  (void)zeus_metrics_from_ticks(ticks.data(), calls.data(), total.data(), n, 256, ms_opt,
+                                tick_hz, &m);
+  unsigned long long sum = 0;
+  for (auto t : ticks)
+    sum += t;
+  const double avg_ticks = n > 0 ? double(sum) / double(n) : 0.0;
+  std::printf("[%s] Kernel wall time: %.3f ms\n", label, ms_opt);
+  std::printf("[%s] Avg cycles/thread: %.3f ; avg ms/thread: %.6f\n", label, avg_ticks,
+              avg_ticks / tick_hz * 1e3);
+  std::printf("[%s] Calls/thread \u2248 %.2f; ms/call \u2248 %.6f\n", label,
+              m.calls_per_thread_mean, m.ms_per_call);
+  std::printf("[%s] Heuristic fraction: %.3f%%\n", label, 100.0 * m.fraction_of_kernel);
+  std::printf("[%s] Serialized sum: %.3f ms; fraction vs kernel: %.3f%%\n", label,
+              double(sum) / tick_hz * 1e3, 100.0 * m.serialized);
+  std::printf("[%s] Block-level fraction p95: %.3f%%\n", label, 100.0 * m.block95);
+  Metrics out;
+  out.label = label;
+  out.ms_per_call = m.ms_per_call;
+  out.calls_per_thread_mean = m.calls_per_thread_mean;
+  out.fraction_of_kernel = m.fraction_of_kernel;
+  out.block95 = m.block95;
+  out.serialized = m.serialized;
+  out.fraction_of_thread = m.fraction_of_thread;
+  return out;
+}
 
 inline void
 hip_check(hipError_t e, const char* what)
@@ -98,7 +149,7 @@ template <class F, int D, Gradient G>
 Result<D>
 run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO_ITER,
     int required, std::string_view fun_name, double tol, int seed, bool parallel,
-    const TrajSpec* tr = nullptr)
+    const TrajSpec* tr = nullptr, int run_id = 0)
 {
   Result<D> res;
   if (N == 0)
@@ -159,6 +210,18 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     HipBuffer<int8_t> near(n);
     StartOut out{st.data(), it.data(), fv.data(), gn.data(), xo.data(), ls.data(), gr.data(),
                  near.data()};
+    // the sequential form times its gradients and updates (the reference's
+    // parallel form reports zero metrics)
+    const int64_t nm = parallel ? 0 : n;
+    HipBuffer<uint64_t> m_at(nm), m_bt(nm), m_tt(nm);
+    HipBuffer<int32_t> m_ac(nm), m_bc(nm);
+    if (!parallel) {
+      out.ad_ticks = m_at.data();
+      out.bfgs_ticks = m_bt.data();
+      out.total_ticks = m_tt.data();
+      out.ad_calls = m_ac.data();
+      out.bfgs_calls = m_bc.data();
+    }
 
     constexpr int NT = front_group_threads<D>();
     constexpr bool kHReg = RowChunk<D, NT>::kPerThread <= 50;
@@ -186,10 +249,12 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     HipBuffer<double> hslab(slab);
 
     hipStream_t s = nullptr;
-    hipEvent_t e0, e1;
+    hipEvent_t ep, e0, e1;
+    hip_check(hipEventCreate(&ep), "event");
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
     init_controls_kernel<<<1, 256, 0, s>>>(ctl.data(), 1, gbx.data(), D);
+    hip_check(hipEventRecord(ep, s), "event");
     if (pso) {
       PsoArgs pa{dP.data(),       1,           0,           nblk,           ctl.data(),
                  X.data(),        V.data(),    pbx.data(),  pbv.data(),     gbx.data(),
@@ -216,8 +281,10 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     summary_kernel<<<1, kSummaryThreads, 0, s>>>(dP.data(), ctl.data(), out, bx.data(), bxo.data());
     hip_check(hipGetLastError(), "summary launch");
     hip_check(hipStreamSynchronize(s), "BFGS runtime");
-    float ms = 0.f;
+    float ms = 0.f, ms_pso = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventElapsedTime(&ms_pso, ep, e0);
+    (void)hipEventDestroy(ep);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 
@@ -234,6 +301,28 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     res.c.stopped = k.stopped;
     res.c.surrendered = k.surrendered;
     res.ms_opt = ms;
+    if (!parallel) {
+      std::vector<uint64_t> at(n), bt(n), tt(n);
+      std::vector<int32_t> ac(n), bc(n);
+      abi_check(zeus_memcpy_d2h(at.data(), m_at.data(), 8 * n), "metrics copy");
+      abi_check(zeus_memcpy_d2h(bt.data(), m_bt.data(), 8 * n), "metrics copy");
+      abi_check(zeus_memcpy_d2h(tt.data(), m_tt.data(), 8 * n), "metrics copy");
+      abi_check(zeus_memcpy_d2h(ac.data(), m_ac.data(), 4 * n), "metrics copy");
+      abi_check(zeus_memcpy_d2h(bc.data(), m_bc.data(), 4 * n), "metrics copy");
+      int dev = 0, khz = 0;
+      hip_check(hipGetDevice(&dev), "hipGetDevice");
+      double hz = 1e8;
+      if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess &&
+          khz > 0)
+        hz = double(khz) * 1e3;
+      res.ad = report_metrics("AD", at, ac, tt, ms, hz);
+      res.bfgs = report_metrics("BFGS", bt, bc, tt, ms, hz);
+    }
+    if (results_table_enabled()) { // zeus.cuh:172-183
+      const double err = util::calculateEuclideanError(std::string(fun_name), res.coordinates, D);
+      util::appendResultsToTsv<std::size_t(D)>(D, int(n), std::string(fun_name), 0.f, ms_pso,
+                                               ms, 0.f, MAX_ITER, PSO_ITER, err, run_id, res);
+    }
     if (t_n > 0) { // zeus.cuh:189-239
       std::vector<double> hc(t_n * D), hf(t_n), hg(t_n), ha(t_n, 0.0);
       std::vector<int8_t> hs(t_n);
@@ -271,7 +360,6 @@ Zeus(Function const& f, double lower, double upper, std::size_t N, int MAX_ITER,
      Gradient gradient = Gradient::Adjoint, int trajectory_steps = -1,
      int trajectory_starts = -1) -> Result<ZEUS_DIM>
 {
-  (void)run;        // labels the call only
   (void)prng;       // every PRNGType draws from the counter-based Philox stream
   (void)nzerosteps; // inert: the line search never returns a zero step
   constexpr int D = int(ZEUS_DIM);
@@ -290,10 +378,10 @@ Zeus(Function const& f, double lower, double upper, std::size_t N, int MAX_ITER,
     if (gradient == Gradient::Adjoint)
       return detail::run<Function, D, Gradient::Adjoint>(f, lower, upper, N, MAX_ITER, PSO_ITER,
                                                          requiredConverged, fun_name, tolerance,
-                                                         seed, parallel, tr);
+                                                         seed, parallel, tr, run);
   return detail::run<Function, D, Gradient::Dual>(f, lower, upper, N, MAX_ITER, PSO_ITER,
                                                   requiredConverged, fun_name, tolerance, seed,
-                                                  parallel, tr);
+                                                  parallel, tr, run);
 }
 
 } // namespace zeus

@@ -87,6 +87,7 @@ struct DeviceState {
   bool init = false;
   int dev = 0;
   int n_cu = 0;
+  double tick_hz = 1e8; // rate of the device's constant clock (wall_clock64)
   hipStream_t stream = nullptr;
   hipEvent_t ev[6] = {};
   void* arena = nullptr;
@@ -141,6 +142,11 @@ ensure_device()
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return ZEUS_E_NODEVICE; // code objects are built for gfx950 only
   g_ds.n_cu = prop.multiProcessorCount;
+  int wall_khz = 0;
+  if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, g_ds.dev) ==
+        hipSuccess &&
+      wall_khz > 0)
+    g_ds.tick_hz = double(wall_khz) * 1e3;
 
   if (hipStreamCreateWithFlags(&g_ds.stream, hipStreamNonBlocking) != hipSuccess)
     return ZEUS_E_LAUNCH;
@@ -344,6 +350,7 @@ struct ClassPlan {
   double* h_slab = nullptr;
   int ctl0 = 0;          // first control block of the class
   bool traced = false;   // some problem records a trajectory
+  bool metrics = false;  // per-start timing requested (ZEUS_FLAG_METRICS)
   // host
   std::vector<ProblemDesc> desc;
   std::vector<int32_t> block_problem;
@@ -383,7 +390,16 @@ carve_class(Carver& cv, ClassPlan& c)
   c.aux = cv.take<double>(c.ops->objective == obj::kGaussian ? np * dev::gaussian_aux_doubles(D)
                                                               : 1);
   c.h_slab = cv.take<double>(size_t(std::max<int64_t>(c.slab, 1)));
-
+  if (c.metrics) {
+    c.out.ad_ticks = cv.take<uint64_t>(n);
+    c.out.bfgs_ticks = cv.take<uint64_t>(n);
+    c.out.total_ticks = cv.take<uint64_t>(n);
+    c.out.ad_calls = cv.take<int32_t>(n);
+    c.out.bfgs_calls = cv.take<int32_t>(n);
+  } else {
+    c.out.ad_ticks = c.out.bfgs_ticks = c.out.total_ticks = nullptr;
+    c.out.ad_calls = c.out.bfgs_calls = nullptr;
+  }
 }
 
 PsoArgs
@@ -594,6 +610,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     }
     c.total = s0;
     c.pso_blocks = b0;
+    c.metrics = (opt.flags & ZEUS_FLAG_METRICS) != 0;
     c.key = TuneKey{c.ops->objective, c.ops->dim, opt.gradient, size_bucket(c.total)};
     c.cands = candidates(c.ops, opt);
     if (c.traced) {
@@ -843,6 +860,36 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       return fail(e);
   }
 
+  // per-start timing of the per-thread form -> metrics per problem
+  std::vector<zeus_metrics> m_ad(n_probs), m_bfgs(n_probs);
+  for (auto& c : classes) {
+    if (!c.metrics || c.cfg.form != ZEUS_FORM_THREAD)
+      continue;
+    for (size_t k = 0; k < c.probs.size(); ++k) {
+      const int64_t s0 = c.desc[k].start0, n = c.desc[k].n_starts;
+      if (n == 0)
+        continue;
+      std::vector<uint64_t> at(n), bt(n), tt(n);
+      std::vector<int32_t> ac(n), bc(n);
+      auto cp = [&](void* dst, const void* src, size_t bytes) {
+        if (e == hipSuccess)
+          e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+      };
+      cp(at.data(), c.out.ad_ticks + s0, 8 * n);
+      cp(bt.data(), c.out.bfgs_ticks + s0, 8 * n);
+      cp(tt.data(), c.out.total_ticks + s0, 8 * n);
+      cp(ac.data(), c.out.ad_calls + s0, 4 * n);
+      cp(bc.data(), c.out.bfgs_calls + s0, 4 * n);
+      if (e != hipSuccess)
+        return fail(e);
+      const int pi = c.probs[k];
+      (void)zeus_metrics_from_ticks(at.data(), ac.data(), tt.data(), n, 256, ms_opt,
+                                    g_ds.tick_hz, &m_ad[pi]);
+      (void)zeus_metrics_from_ticks(bt.data(), bc.data(), tt.data(), n, 256, ms_opt,
+                                    g_ds.tick_hz, &m_bfgs[pi]);
+    }
+  }
+
   // ---- results ------------------------------------------------------------
   // best Result per problem: fetch its status/iter/fval/gnorm
   const double ms_total =
@@ -870,6 +917,8 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     r.ms_pso = ms_pso;
     r.ms_opt = ms_opt;
     r.ms_total = ms_total;
+    r.ad = m_ad[i];
+    r.bfgs = m_bfgs[i];
     if (best_x)
       std::memcpy(best_x + bx_user[i], h_bx + bx_user[i], sizeof(double) * probs[i].dim);
   }
