@@ -169,7 +169,7 @@ struct DistMatrix {
   ZEUS_D ZEUS_FI void pass(const double* x, const double* s, const double* a,
                            bool pending, double* part)
   {
-    if constexpr (!REG) {
+    if constexpr (!REG && D > 256) {
       if (pending)
         pass_slab<true>(x, s, a, part);
       else
@@ -179,18 +179,19 @@ struct DistMatrix {
     }
   }
 
+  // Register form, and slab form for rows up to 256 elements.
   ZEUS_D ZEUS_FI void pass_regs(const double* x, const double* s, const double* a,
                                 bool pending, double* part)
   {
     const int c = threadIdx.x % Lay::CH, r0 = threadIdx.x / Lay::CH;
-#pragma unroll 64
+#pragma unroll(REG ? 64 : 1)
     for (int k = 0; k < Lay::R; ++k) {
       const int row = r0 + k * Lay::RPP;
       if (row < D) {
         const double sr = pending ? s[row] : 0.0;
         const double ar = pending ? a[row] : 0.0;
         double acc = 0.0;
-#pragma unroll 64
+#pragma unroll(REG ? 64 : 1)
         for (int l = 0; l < Lay::L; ++l) {
           const int col = c * Lay::L + l;
           if (col < D) {
@@ -207,11 +208,14 @@ struct DistMatrix {
     }
   }
 
-  // Slab form of pass(): the same arithmetic in the same order, with the
-  // columns taken in batches of kBatch whose slab loads are all issued before
-  // any of them is used (a branch per element would wait out the memory
-  // latency element by element).  Padding columns (col >= D) are loaded but
-  // neither updated nor summed.
+  // Slab form of pass() for long rows (D > 256): the same arithmetic in the
+  // same order, with the columns taken in batches of kBatch whose slab loads
+  // are all issued before any of them is used (a branch per element would
+  // wait out the memory latency element by element, and at these sizes one
+  // wave per SIMD has no other wave to hide it; measured 2x on the 321-D
+  // MLP).  Shorter rows keep the element loop of pass_regs, which leaves
+  // registers for more resident waves (measured faster at D = 150).  Padding
+  // columns (col >= D) are loaded but neither updated nor summed.
   template <bool PENDING>
   ZEUS_D ZEUS_FI void pass_slab(const double* x, const double* s, const double* a, double* part)
   {
@@ -227,6 +231,9 @@ struct DistMatrix {
       double* hrow = base + int64_t(k) * Lay::L * NT;
       double acc = 0.0;
       int l0 = 0;
+      // one batch per step: unrolled further, the loads of every batch are
+      // hoisted together and the register count (and occupancy) suffers
+#pragma unroll 1
       for (; l0 + kBatch <= Lay::L; l0 += kBatch) {
         double h[kBatch];
 #pragma unroll
@@ -250,6 +257,7 @@ struct DistMatrix {
             hrow[int64_t(l0 + i) * NT] = h[i];
         }
       }
+#pragma unroll 1
       for (int l = l0; l < Lay::L; ++l) {
         const int col = c * Lay::L + l;
         if (col < D) {
