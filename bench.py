@@ -81,7 +81,8 @@ def cpu_baseline(threads: int):
     from oracle import binding as orc
     C = gauss_matrix().reshape(-1)
     t0 = time.perf_counter()
-    pbx, _, _, _ = orc.pso(orc.GAUSSIAN, D, -5.0, 5.0, N, PSO_ITER, SEED, params=C)
+    pbx, _, _, _ = orc.pso(orc.GAUSSIAN, D, -5.0, 5.0, N, PSO_ITER, SEED, params=C,
+                           prng=orc.PRNG_PHILOX)
     t_pso = time.perf_counter() - t0
     K = 2 * threads
     t0 = time.perf_counter()
@@ -123,7 +124,7 @@ def main() -> int:
 
     def call(grad, required):
         return z.Zeus(f, -5.0, 5.0, N, MAX_ITER, PSO_ITER, required, "gaussian", TOL, SEED, 0,
-                      gradient=grad)
+                      prng=z.PRNGType.PHILOX, gradient=grad)
 
     def timed(grad, required, steps, warmup):
         for _ in range(warmup):
@@ -173,7 +174,8 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "gaussian-150 multistart: 4096 starts, PSO 10, MAX_ITER 10000, "
-                               "tol 1e-8, required 100, seed 42, adjoint gradient",
+                               "tol 1e-8, required 100, seed 42, adjoint gradient, "
+                               "Philox generator (PRNGType.PHILOX)",
                    "dim": D, "starts": N, "required": REQUIRED, "pso_iter": PSO_ITER,
                    "parallelism": layout},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,

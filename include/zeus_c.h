@@ -57,6 +57,15 @@ typedef int32_t zeus_status_t;
 #define ZEUS_OBJ_MLP 6        /* params: x[m] y[m]; dim 321                */
 #define ZEUS_OBJ_SPHERE 7     /* params: none                              */
 
+/* Generators (reference PRNGType, include/traits.hpp:67-71), one stream of
+   uniforms in (0, 1] per start: XORWOW (Marsaglia 2003, a state per start
+   seeded from (seed, stream, start)), PHILOX (counter-based Philox4x32-10
+   keyed by (seed, stream)), SOBOL (dimension 1 of Sobol's sequence read from
+   index start on; seed unused).  None reproduces cuRAND's streams. */
+#define ZEUS_PRNG_XORWOW 0
+#define ZEUS_PRNG_PHILOX 1
+#define ZEUS_PRNG_SOBOL 2
+
 #define ZEUS_GRAD_ADJOINT 0
 #define ZEUS_GRAD_DUAL 1
 
@@ -111,7 +120,7 @@ typedef struct zeus_problem {
   int32_t required;    /* requiredConverged (<= 0: never stop early)       */
   uint32_t seed;       /* Philox key word 0                                */
   uint32_t stream;     /* Philox key word 1 (distinct streams per problem) */
-  int32_t reserved;
+  int32_t prng;        /* ZEUS_PRNG_* (0: XORWOW, the reference default)   */
   double tol;          /* gradient-norm tolerance                          */
   const double* params;/* device pointer, objective data (see ZEUS_OBJ_*)  */
   int64_t n_params;
@@ -223,6 +232,11 @@ double zeus_philox_uniform(uint32_t seed, uint32_t stream, uint64_t start,
 zeus_status_t zeus_philox_fill(double* out, int64_t n, uint32_t seed,
                                uint32_t stream, uint64_t start,
                                uint64_t draw0, void* stream_handle);
+
+/* Device fill of draws 0..n-1 of one start's stream under generator prng
+   (ZEUS_PRNG_*), in the order the optimizer consumes them. */
+zeus_status_t zeus_stream_fill(double* out, int64_t n, int32_t prng, uint32_t seed,
+                               uint32_t stream, uint64_t start, void* stream_handle);
 
 /* Device memory (replaces CudaBuffer's cudaMalloc/cudaMemcpy,
    include/cuda_buffer.hpp:14-209). */

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
 
 ROSENBROCK, RASTRIGIN, ACKLEY, GAUSSIAN, LOGLIK, MLP, SPHERE = 1, 2, 3, 4, 5, 6, 7
 GRAD_ADJOINT, GRAD_DUAL = 0, 1
+PRNG_XORWOW, PRNG_PHILOX, PRNG_SOBOL = 0, 1, 2
 UPDATE_RANK2, UPDATE_LITERAL = 0, 1
 
 D_ADD, D_SUB, D_MUL, D_DIV, D_NEG, D_ABS, D_SIN, D_COS, D_EXP, D_SQRT, \
@@ -44,7 +45,7 @@ class Problem(C.Structure):
                 ("required", C.c_int), ("max_iter", C.c_int),
                 ("pso_iter", C.c_int), ("seed", C.c_uint32),
                 ("stream", C.c_uint32), ("tol", C.c_double), ("params", _dp),
-                ("n_params", C.c_int64), ("x0", _dp)]
+                ("n_params", C.c_int64), ("x0", _dp), ("prng", C.c_int)]
 
 
 class Traj(C.Structure):
@@ -110,6 +111,15 @@ def lib():
         L.orc_pso.argtypes = [C.POINTER(Problem), _dp, _dp, _dp, _dp]
         L.orc_pso_step.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
                                    C.c_double, _dp, _dp, _dp, _dp, _dp]
+        L.orc_xorwow_seed.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64,
+                                      C.POINTER(C.c_uint32)]
+        L.orc_xorwow_next.restype = C.c_uint32
+        L.orc_xorwow_next.argtypes = [C.POINTER(C.c_uint32)]
+        L.orc_sobol_uniform.restype = C.c_double
+        L.orc_sobol_uniform.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_stream_uniforms.restype = C.c_int
+        L.orc_stream_uniforms.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint64,
+                                          C.c_int64, _dp]
         L.orc_argmin.restype = C.c_int64
         L.orc_argmin.argtypes = [_dp, C.c_int64]
         _lib = L
@@ -240,21 +250,48 @@ class RunResult:
 
 
 def _problem(objective, dim, lower, upper, n_starts, required, max_iter, pso_iter,
-             seed, stream, tol, params, x0):
+             seed, stream, tol, params, x0, prng=PRNG_XORWOW):
     return Problem(objective, dim, lower, upper, n_starts, required, max_iter,
                    pso_iter, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF, tol,
-                   _ptr(params), 0 if params is None else params.size, _ptr(x0))
+                   _ptr(params), 0 if params is None else params.size, _ptr(x0), prng)
+
+
+def xorwow_seed(seed, stream, start):
+    st = (C.c_uint32 * 6)()
+    lib().orc_xorwow_seed(seed & 0xFFFFFFFF, stream & 0xFFFFFFFF, start, st)
+    return list(st)
+
+
+def xorwow_next(state):
+    """One xorwow step on a 6-word state list (updated in place)."""
+    st = (C.c_uint32 * 6)(*state)
+    v = lib().orc_xorwow_next(st)
+    state[:] = list(st)
+    return int(v)
+
+
+def sobol_uniform(start, draw):
+    return lib().orc_sobol_uniform(start, draw)
+
+
+def stream_uniforms(prng, seed, stream, start, n):
+    out = np.zeros(n)
+    rc = lib().orc_stream_uniforms(prng, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF, start, n,
+                                   _ptr(out))
+    if rc != 0:
+        raise ValueError("bad generator arguments")
+    return out
 
 
 def run(objective, dim, lower, upper, n_starts, max_iter, pso_iter, required, tol,
         seed, *, stream=0, params=None, x0=None, grad=GRAD_ADJOINT,
         update_mode=UPDATE_RANK2, threads=0, early_stop=True, trajectory=None,
-        trajectory_alpha=True) -> RunResult:
+        trajectory_alpha=True, prng=PRNG_XORWOW) -> RunResult:
     """trajectory=(steps, starts) records the BFGS steps (orc_traj layout)."""
     params = None if params is None else _f64(params)
     x0 = None if x0 is None else _f64(x0).reshape(n_starts, dim)
     P = _problem(objective, dim, lower, upper, n_starts, required, max_iter,
-                 pso_iter, seed, stream, tol, params, x0)
+                 pso_iter, seed, stream, tol, params, x0, prng)
     tarr = None
     tptr = None
     if trajectory is not None:
@@ -289,10 +326,10 @@ def run(objective, dim, lower, upper, n_starts, max_iter, pso_iter, required, to
 
 
 def pso(objective, dim, lower, upper, n_starts, pso_iter, seed, *, stream=0,
-        params=None):
+        params=None, prng=PRNG_XORWOW):
     params = None if params is None else _f64(params)
     P = _problem(objective, dim, lower, upper, n_starts, 0, 0, pso_iter, seed,
-                 stream, 0.0, params, None)
+                 stream, 0.0, params, None, prng)
     pbx = np.zeros((n_starts, dim))
     pbv = np.zeros(n_starts)
     gbx = np.zeros(dim)

@@ -68,13 +68,123 @@ orc_uniform(uint32_t seed, uint32_t stream, uint64_t start, uint64_t draw)
   return (double)(m + 1u) * 0x1p-53;
 }
 
+/* ---- selectable generators (include/traits.hpp:67-71) ----------------
+   XORWOW: Marsaglia, "Xorshift RNGs", J. Stat. Softw. 8(14), 2003; the
+   state of a start is seeded from Philox counter blocks 2^64-1 and 2^64-2
+   (never used by draws); a uniform takes two outputs, (m + 1) 2^-53 with
+   m = top 32 bits of the first and top 21 of the second.
+   SOBOL: dimension 1 (direction numbers 2^(31-j)): point n is
+   bitreverse32(n ^ (n >> 1)); start i reads from point i on; (x + 1) 2^-32. */
+void
+orc_xorwow_seed(uint32_t seed, uint32_t stream, uint64_t start, uint32_t st[6])
+{
+  uint32_t key[2] = {seed, stream};
+  uint32_t ca[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, (uint32_t)start, (uint32_t)(start >> 32)};
+  uint32_t cb[4] = {0xFFFFFFFEu, 0xFFFFFFFFu, (uint32_t)start, (uint32_t)(start >> 32)};
+  uint32_t a[4], b[4];
+  orc_philox4x32_10(ca, key, a);
+  orc_philox4x32_10(cb, key, b);
+  st[0] = a[0];
+  st[1] = a[1];
+  st[2] = a[2];
+  st[3] = a[3];
+  st[4] = b[0];
+  st[5] = b[1];
+  if ((st[0] | st[1] | st[2] | st[3] | st[4]) == 0u)
+    st[4] = 0x2545F491u;
+}
+
+uint32_t
+orc_xorwow_next(uint32_t st[6])
+{
+  uint32_t t = st[0] ^ (st[0] >> 2);
+  st[0] = st[1];
+  st[1] = st[2];
+  st[2] = st[3];
+  st[3] = st[4];
+  st[4] = (st[4] ^ (st[4] << 4)) ^ (t ^ (t << 1));
+  st[5] += 362437u;
+  return st[4] + st[5];
+}
+
+static double
+xorwow_uniform(uint32_t st[6])
+{
+  uint64_t hi = orc_xorwow_next(st);
+  uint64_t lo = orc_xorwow_next(st);
+  uint64_t m = (hi << 21) | (lo >> 11);
+  return (double)(m + 1u) * 0x1p-53;
+}
+
+double
+orc_sobol_uniform(uint64_t start, uint64_t draw)
+{
+  uint32_t n = (uint32_t)(start + draw);
+  uint32_t v = n ^ (n >> 1), r = 0;
+  for (int b = 0; b < 32; ++b)
+    r |= ((v >> b) & 1u) << (31 - b);
+  return ((double)r + 1.0) * 0x1p-32;
+}
+
+/* One start's stream: draws are requested in increasing index order. */
+typedef struct {
+  int kind;
+  uint32_t seed, stream;
+  uint64_t start, next;
+  uint32_t xs[6];
+} orc_stream;
+
+static void
+stream_init(orc_stream* r, const orc_problem* P, uint64_t start)
+{
+  r->kind = P->prng;
+  r->seed = P->seed;
+  r->stream = P->stream;
+  r->start = start;
+  r->next = 0;
+  if (r->kind == ORC_PRNG_XORWOW)
+    orc_xorwow_seed(P->seed, P->stream, start, r->xs);
+}
+
+static double
+stream_uniform(orc_stream* r, uint64_t draw)
+{
+  if (r->kind == ORC_PRNG_PHILOX)
+    return orc_uniform(r->seed, r->stream, r->start, draw);
+  if (r->kind == ORC_PRNG_SOBOL)
+    return orc_sobol_uniform(r->start, draw);
+  while (r->next < draw) {
+    (void)xorwow_uniform(r->xs);
+    ++r->next;
+  }
+  ++r->next;
+  return xorwow_uniform(r->xs);
+}
+
 /* include/utils.cuh:353-358: lower + (upper + (-lower)) * u */
 static double
-uniform_in(uint32_t seed, uint32_t stream, uint64_t start, uint64_t draw,
-           double lo, double hi)
+uniform_in(orc_stream* r, uint64_t draw, double lo, double hi)
 {
-  double u = orc_uniform(seed, stream, start, draw);
+  double u = stream_uniform(r, draw);
   return lo + (hi + (-lo)) * u;
+}
+
+int
+orc_stream_uniforms(int prng, uint32_t seed, uint32_t stream, uint64_t start, int64_t n,
+                    double* out)
+{
+  if (prng < 0 || prng > 2 || n < 0 || (n > 0 && !out))
+    return -1;
+  orc_problem P;
+  memset(&P, 0, sizeof(P));
+  P.prng = prng;
+  P.seed = seed;
+  P.stream = stream;
+  orc_stream r;
+  stream_init(&r, &P, start);
+  for (int64_t k = 0; k < n; ++k)
+    out[k] = stream_uniform(&r, (uint64_t)k);
+  return 0;
 }
 
 /* ======================================================================
@@ -863,17 +973,23 @@ pso_run(const orc_problem* P, int nt, double* pbx, double* pbv, double* gbx,
     free(V);
     return -2;
   }
+  orc_stream* rs = (orc_stream*)malloc(sizeof(orc_stream) * (size_t)(N > 0 ? N : 1));
+  if (!rs) {
+    free(X);
+    free(V);
+    return -2;
+  }
   *gbv = INFINITY;
   for (int d = 0; d < D; ++d)
     gbx[d] = 0.0;
 
 #pragma omp parallel for schedule(static) num_threads(nt)
   for (int64_t i = 0; i < N; ++i) {
+    stream_init(&rs[i], P, (uint64_t)i);
     for (int d = 0; d < D; ++d) {
       size_t o = (size_t)i * D + d;
-      double rx = uniform_in(P->seed, P->stream, (uint64_t)i, 2u * d, lo, hi);
-      double rv =
-        uniform_in(P->seed, P->stream, (uint64_t)i, 2u * d + 1u, -vr, vr);
+      double rx = uniform_in(&rs[i], 2u * d, lo, hi);
+      double rv = uniform_in(&rs[i], 2u * d + 1u, -vr, vr);
       X[o] = rx;
       V[o] = rv;
       pbx[o] = rx;
@@ -888,10 +1004,8 @@ pso_run(const orc_problem* P, int nt, double* pbx, double* pbv, double* gbx,
     for (int64_t i = 0; i < N; ++i) {
       for (int d = 0; d < D; ++d) {
         size_t o = (size_t)i * D + d;
-        double r1 =
-          uniform_in(P->seed, P->stream, (uint64_t)i, base + 2u * d, 0.0, 1.0);
-        double r2 = uniform_in(P->seed, P->stream, (uint64_t)i,
-                               base + 2u * d + 1u, 0.0, 1.0);
+        double r1 = uniform_in(&rs[i], base + 2u * d, 0.0, 1.0);
+        double r2 = uniform_in(&rs[i], base + 2u * d + 1u, 0.0, 1.0);
         double xi = X[o], vi = V[o], pb = pbx[o], gb = gbx[d];
         double nv = w * vi + c1 * r1 * (pb - xi) + c2 * r2 * (gb - xi);
         V[o] = nv;
@@ -905,6 +1019,7 @@ pso_run(const orc_problem* P, int nt, double* pbx, double* pbv, double* gbx,
     }
     pso_gbest(pbv, pbx, N, D, gbv, gbx);
   }
+  free(rs);
   free(X);
   free(V);
   return 0;
@@ -966,10 +1081,11 @@ run_start(const orc_problem* P, const orc_options* O, int64_t s,
   for (int i = 0; i < D; ++i)
     for (int j = 0; j < D; ++j)
       H[i * D + j] = (i == j) ? 1.0 : 0.0;
+  orc_stream rs;
+  stream_init(&rs, P, (uint64_t)s);
   for (int d = 0; d < D; ++d) {
     x[d] = starts ? starts[(size_t)s * D + d]
-                  : uniform_in(P->seed, P->stream, (uint64_t)s, (uint64_t)d,
-                               P->lower, P->upper);
+                  : uniform_in(&rs, (uint64_t)d, P->lower, P->upper);
     g[d] = 0.0;
   }
   double f0 = orc_value(&f, x);

@@ -49,6 +49,8 @@ enum {
 };
 
 enum { ORC_GRAD_ADJOINT = 0, ORC_GRAD_DUAL = 1 };
+/* generators (include/traits.hpp:67-71) */
+enum { ORC_PRNG_XORWOW = 0, ORC_PRNG_PHILOX = 1, ORC_PRNG_SOBOL = 2 };
 enum { ORC_UPDATE_RANK2 = 0, ORC_UPDATE_LITERAL = 1 };
 
 /* ---- counter-based random numbers ------------------------------------ */
@@ -115,7 +117,16 @@ typedef struct {
   const double* params;
   int64_t n_params;
   const double* x0; /* optional n_starts x dim start points (skips PSO)  */
+  int prng;         /* ORC_PRNG_*                                        */
 } orc_problem;
+
+/* Generators: XORWOW state seeding and step, SOBOL dimension-1 draw, and n
+   consecutive uniforms of one start's stream (returns 0, -1 on bad args). */
+void orc_xorwow_seed(uint32_t seed, uint32_t stream, uint64_t start, uint32_t st[6]);
+uint32_t orc_xorwow_next(uint32_t st[6]);
+double orc_sobol_uniform(uint64_t start, uint64_t draw);
+int orc_stream_uniforms(int prng, uint32_t seed, uint32_t stream, uint64_t start, int64_t n,
+                        double* out);
 
 /* Trajectory recording (include/zeus.cuh:62-103, bfgs_sequential.cuh:
    100-109, bfgs_parallel.cuh:155-184, bfgs_common.cuh:108-238): for step

@@ -18,6 +18,7 @@ OBJ_ROSENBROCK, OBJ_RASTRIGIN, OBJ_ACKLEY, OBJ_GAUSSIAN, OBJ_LOGLIK, OBJ_MLP, OB
 GRAD_ADJOINT, GRAD_DUAL = 0, 1
 FORM_AUTO, FORM_THREAD, FORM_GROUP, FORM_TILE = 0, 1, 2, 3
 FLAG_METRICS = 1
+PRNG_XORWOW, PRNG_PHILOX, PRNG_SOBOL = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
@@ -33,7 +34,7 @@ class Problem(C.Structure):
     _fields_ = [("objective", C.c_int32), ("dim", C.c_int32), ("lower", C.c_double),
                 ("upper", C.c_double), ("n_starts", C.c_int64), ("max_iter", C.c_int32),
                 ("pso_iter", C.c_int32), ("required", C.c_int32), ("seed", C.c_uint32),
-                ("stream", C.c_uint32), ("reserved", C.c_int32), ("tol", C.c_double),
+                ("stream", C.c_uint32), ("prng", C.c_int32), ("tol", C.c_double),
                 ("params", C.c_void_p), ("n_params", C.c_int64), ("x0", C.c_void_p),
                 ("name", C.c_char_p), ("trajectory", C.POINTER(Trajectory))]
 
@@ -77,6 +78,8 @@ SIGNATURES = {
                                             C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                             C.POINTER(Result), _dp]),
+    "zeus_stream_fill": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, C.c_uint32, C.c_uint32,
+                                     C.c_uint64, C.c_void_p]),
     "zeus_write_trajectory": (C.c_int32, [C.c_char_p, _dp, _dp, _dp, C.c_void_p, _dp,
                                           C.c_int32, C.c_int32, C.c_int32]),
     "zeus_pso_run": (C.c_int32, [C.POINTER(Problem), C.c_void_p, C.c_void_p, C.c_void_p,

@@ -25,12 +25,15 @@ __all__ = [
     "Rastrigin", "Ackley", "Sphere", "Gaussian", "LogLikelihood", "Mlp", "Zeus",
     "Problem", "run_batch", "pso", "evaluate", "argmin", "philox_uniform", "philox_fill",
     "make_dijet", "make_mlp_data", "make_spd", "tuned_config", "set_fault_hooks",
-    "write_trajectory", "euclidean_error", "append_results_tsv",
+    "write_trajectory", "euclidean_error", "append_results_tsv", "stream_uniforms",
 ]
 
 
 class PRNGType(enum.IntEnum):
-    """Reference traits.hpp:67-71; every value selects the Philox stream."""
+    """Reference traits.hpp:67-71 (values = ZEUS_PRNG_*): XORWOW (Marsaglia's
+    xorwow, a state per start), PHILOX (counter-based Philox4x32-10), SOBOL
+    (dimension 1 of Sobol's sequence from index start on).  Own seeding and
+    conversions: none reproduces cuRAND's streams."""
     XORWOW = 0
     PHILOX = 1
     SOBOL = 2
@@ -215,6 +218,7 @@ class Problem:
     # `starts` starts (zeus_trajectory); alpha: also the line-search steps
     trajectory: tuple | None = None
     trajectory_alpha: bool = True
+    prng: int = PRNGType.XORWOW
 
 
 class _TrajBuffers:
@@ -287,7 +291,8 @@ def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group
             keep.append(tptr)
         P[i] = capi.Problem(pr.f.kind, pr.f.dim, float(pr.lower), float(pr.upper), int(pr.N),
                             int(pr.MAX_ITER), int(pr.PSO_ITER), int(pr.requiredConverged),
-                            int(pr.seed) & 0xFFFFFFFF, int(pr.stream) & 0xFFFFFFFF, 0,
+                            int(pr.seed) & 0xFFFFFFFF, int(pr.stream) & 0xFFFFFFFF,
+                            int(pr.prng),
                             float(pr.tolerance), addr or None, nparam, x0addr or None, name,
                             tptr)
     opt = _options(gradient, form, True, group_threads, 0, use_graph, metrics)
@@ -348,7 +353,7 @@ def Zeus(f: Objective, lower: float, upper: float, N: int, MAX_ITER: int, PSO_IT
     Argument order and meaning follow zeus::Zeus (include/zeus.cuh:341-358).
     ``parallel`` selects the per-workgroup form (True) or the per-thread form
     (False) unless ``form`` overrides it; ``run`` only labels the call;
-    ``prng`` is accepted for API parity (every value draws from Philox).
+    ``prng`` selects the generator of the starts' streams (PRNGType).
     ``trajectory_file`` (".tsv") records every start's iterations and writes
     them as reference zeus.cuh:189-239 does, the line-search steps only for
     ``parallel`` (the reference's sequential form writes zeros); the arrays
@@ -369,7 +374,7 @@ def Zeus(f: Objective, lower: float, upper: float, N: int, MAX_ITER: int, PSO_IT
         starts = N if trajectory_starts is None else min(int(trajectory_starts), N)
         traj = (steps, starts)
     pr = Problem(f, lower, upper, N, MAX_ITER, PSO_ITER, requiredConverged, tolerance, seed,
-                 fun_name, stream, x0, traj, bool(parallel))
+                 fun_name, stream, x0, traj, bool(parallel), int(prng))
     if form is None:
         form = "group" if parallel else "thread"
     res = run_batch([pr], gradient=gradient, form=form, group_threads=group_threads,
@@ -385,6 +390,16 @@ def Zeus(f: Objective, lower: float, upper: float, N: int, MAX_ITER: int, PSO_IT
         except capi.ZeusError as e:
             warnings.warn(f"trajectory not written to {trajectory_file}: {e}")
     return res
+
+
+def stream_uniforms(prng, seed: int, stream: int, start: int, n: int) -> np.ndarray:
+    """Draws 0..n-1 of one start's stream, generated on the device
+    (zeus_stream_fill)."""
+    out = DeviceArray(nbytes=8 * n)
+    check(lib().zeus_stream_fill(out.ptr, int(n), int(prng), int(seed) & 0xFFFFFFFF,
+                                 int(stream) & 0xFFFFFFFF, int(start), None),
+          "zeus_stream_fill")
+    return out.to_host()
 
 
 def euclidean_error(fun_name: str, x) -> float:
@@ -433,11 +448,12 @@ def write_trajectory(filename: str, traj: dict) -> None:
 
 
 # --- phases and utilities -------------------------------------------------------
-def pso(f: Objective, lower, upper, N, PSO_ITER, seed, stream=0):
+def pso(f: Objective, lower, upper, N, PSO_ITER, seed, stream=0, prng=PRNGType.XORWOW):
     """PSO alone (zeus_pso_run): returns pBestX, pBestVal, gBestX, gBestVal."""
     addr, nparam = f.device_params()
     P = capi.Problem(f.kind, f.dim, float(lower), float(upper), int(N), 0, int(PSO_ITER), 0,
-                     int(seed) & 0xFFFFFFFF, int(stream) & 0xFFFFFFFF, 0, 0.0, addr or None,
+                     int(seed) & 0xFFFFFFFF, int(stream) & 0xFFFFFFFF, int(prng), 0.0,
+                     addr or None,
                      nparam, None, None, None)
     px = DeviceArray(nbytes=8 * N * f.dim)
     pv = DeviceArray(nbytes=8 * N)

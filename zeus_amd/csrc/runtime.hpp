@@ -5,6 +5,7 @@
 #include "zeus/bfgs_group.hpp"
 #include "zeus/bfgs_thread.hpp"
 #include "zeus/bfgs_tile.hpp"
+#include "zeus/prng.hpp"
 #include "zeus/pso.hpp"
 #include "zeus/summary.hpp"
 

@@ -40,7 +40,7 @@ struct ProblemDesc {
   int32_t block0;   // first PSO workgroup of this problem
   int32_t min_kind; // known minimum for the "actual" count: 0 origin,
                     // 1 all ones (rosenbrock), 2 (0, -1) (goldstein_price)
-  int32_t pad0;
+  int32_t prng;     // generator (prng.hpp: 0 XORWOW, 1 PHILOX, 2 SOBOL)
   uint32_t seed;    // Philox key word 0
   uint32_t stream;  // Philox key word 1
   double tol;

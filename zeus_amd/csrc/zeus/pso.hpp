@@ -45,6 +45,10 @@ struct PsoArgs {
   double* part_x;      // [2][n_blocks][D]: its coordinates
   double* gstate;      // n_probs x [2][1 + D]: gBest (value, x) by iteration parity
   const int32_t* block_problem; // workgroup -> problem
+  // XORWOW / SOBOL streams: this iteration's uniforms, draws[k * draws_stride
+  // + s] for k = 2d + j (prng.hpp); nullptr: every problem draws Philox
+  const double* draws;
+  int64_t draws_stride;
 };
 
 // (value, index) order for argmin: NaN loses to any number, ties and the
@@ -151,26 +155,33 @@ pso_fold(const PsoArgs& a, int pi, const ProblemDesc& P, int t_prev, double* gb,
 
 // Position/velocity of one coordinate: init (t = 0) or update (t >= 1).
 // Returns the new position.
+// Uniform j of coordinate d of start s in this iteration, mapped to
+// [lo, hi] as reference include/utils.cuh:353-358 does.
+ZEUS_D ZEUS_FI double
+pso_uniform(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d, int j, double lo,
+            double hi)
+{
+  const double u = a.draws ? a.draws[int64_t(2 * d + j) * a.draws_stride + s]
+                           : rng::uniform(P.seed, P.stream, uint64_t(s - P.start0),
+                                          rng::pso_draw(D, a.iter, d, j));
+  return lo + (hi + (-lo)) * u;
+}
+
 ZEUS_D ZEUS_FI double
 pso_coord(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d, const double* gb)
 {
-  const int64_t sl = s - P.start0;
   const int64_t o = s * D + d;
   if (a.iter == 0) {
     const double vr = (P.upper - P.lower) * 0.1;
-    const double rx = rng::uniform_in(P.seed, P.stream, uint64_t(sl),
-                                      rng::pso_draw(D, 0, d, 0), P.lower, P.upper);
-    const double rv =
-      rng::uniform_in(P.seed, P.stream, uint64_t(sl), rng::pso_draw(D, 0, d, 1), -vr, vr);
+    const double rx = pso_uniform(a, P, D, s, d, 0, P.lower, P.upper);
+    const double rv = pso_uniform(a, P, D, s, d, 1, -vr, vr);
     a.X[o] = rx;
     a.V[o] = rv;
     a.pbx[o] = rx;
     return rx;
   }
-  const double r1 = rng::uniform_in(P.seed, P.stream, uint64_t(sl),
-                                    rng::pso_draw(D, a.iter, d, 0), 0.0, 1.0);
-  const double r2 = rng::uniform_in(P.seed, P.stream, uint64_t(sl),
-                                    rng::pso_draw(D, a.iter, d, 1), 0.0, 1.0);
+  const double r1 = pso_uniform(a, P, D, s, d, 0, 0.0, 1.0);
+  const double r2 = pso_uniform(a, P, D, s, d, 1, 0.0, 1.0);
   const double xi = a.X[o], vi = a.V[o], pb = a.pbx[o];
   const double nv = kPsoW * vi + kPsoC1 * r1 * (pb - xi) + kPsoC2 * r2 * (gb[d] - xi);
   a.V[o] = nv;

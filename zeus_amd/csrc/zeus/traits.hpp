@@ -62,9 +62,10 @@ struct HasAdjointT<F, VoidT<decltype(std::declval<const F&>().value_and_gradient
 template <typename F>
 inline constexpr bool HasAdjoint = HasAdjointT<F>::value;
 
-// PRNG algorithms (reference traits.hpp:67-71).  All three names are
-// accepted; every one maps to the counter-based Philox stream (the cuRAND
-// XORWOW and Sobol streams are library-internal and not reproduced).
+// PRNG algorithms (reference traits.hpp:67-71): Marsaglia's xorwow with a
+// state per start, counter-based Philox4x32-10, and dimension 1 of Sobol's
+// sequence read from the start's index on (prng.hpp).  Seeding and
+// conversions are this library's own: cuRAND's streams are not reproduced.
 enum class PRNGType { XORWOW, PHILOX, SOBOL };
 
 // Gradient method of the BFGS.

@@ -31,6 +31,7 @@
 #include "hip_buffer.hpp"
 #include "matrix.hpp"
 #include "objectives.hpp"
+#include "prng.hpp"
 #include "pso.hpp"
 #include "report.hpp"
 #include "result.hpp"
@@ -149,7 +150,7 @@ template <class F, int D, Gradient G>
 Result<D>
 run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO_ITER,
     int required, std::string_view fun_name, double tol, int seed, bool parallel,
-    const TrajSpec* tr = nullptr, int run_id = 0)
+    const TrajSpec* tr = nullptr, int run_id = 0, int prng = rng::kXorwow)
 {
   Result<D> res;
   if (N == 0)
@@ -191,6 +192,14 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     P.stream = 0;
     P.tol = tol;
     P.x0 = pso ? pbx.data() : nullptr;
+    // XORWOW / SOBOL: per-start states, one PSO iteration's draws, or the
+    // generated BFGS starts (prng.hpp); PHILOX draws in the kernels
+    P.prng = prng;
+    const bool stateful = prng != rng::kPhilox;
+    HipBuffer<uint32_t> rstate(stateful ? 6 * n : 0);
+    HipBuffer<double> rdraws(stateful && pso ? 2 * n * D : 0), rstarts(stateful && !pso ? n * D : 0);
+    if (stateful && !pso)
+      P.x0 = rstarts.data();
     const int64_t t_steps = tr ? tr->steps : 0, t_starts = tr ? tr->starts : 0;
     const int64_t t_n = t_steps * t_starts;
     HipBuffer<double> t_coords(t_n * D), t_fval(t_n), t_grad(t_n),
@@ -254,13 +263,22 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
     init_controls_kernel<<<1, 256, 0, s>>>(ctl.data(), 1, gbx.data(), D);
+    if (stateful) {
+      rng_init_kernel<<<nblk, 256, 0, s>>>(dP.data(), 1, n, rstate.data());
+      if (!pso)
+        rng_starts_kernel<<<nblk, 256, 0, s>>>(dP.data(), 1, n, rstate.data(), rstarts.data());
+    }
     hip_check(hipEventRecord(ep, s), "event");
     if (pso) {
       PsoArgs pa{dP.data(),       1,           0,           nblk,           ctl.data(),
                  X.data(),        V.data(),    pbx.data(),  pbv.data(),     gbx.data(),
-                 part_val.data(), part_idx.data(), part_x.data(), gstate.data(), bprob.data()};
+                 part_val.data(), part_idx.data(), part_x.data(), gstate.data(), bprob.data(),
+                 stateful ? rdraws.data() : nullptr, n};
       for (int t = 0; t <= PSO_ITER; ++t) {
         pa.iter = t;
+        if (stateful)
+          rng_pso_draws_kernel<<<nblk, 256, 0, s>>>(dP.data(), 1, n, t, rstate.data(),
+                                                    rdraws.data());
         pso_lane_kernel<F, D, UserProvider<F>><<<nblk, 256, 0, s>>>(pa, prov);
       }
     }
@@ -360,7 +378,9 @@ Zeus(Function const& f, double lower, double upper, std::size_t N, int MAX_ITER,
      Gradient gradient = Gradient::Adjoint, int trajectory_steps = -1,
      int trajectory_starts = -1) -> Result<ZEUS_DIM>
 {
-  (void)prng;       // every PRNGType draws from the counter-based Philox stream
+  const int kind = prng == PRNGType::PHILOX ? rng::kPhilox
+                   : prng == PRNGType::SOBOL ? rng::kSobol
+                                             : rng::kXorwow;
   (void)nzerosteps; // inert: the line search never returns a zero step
   constexpr int D = int(ZEUS_DIM);
   // trajectory_file: every start's iterations, written as .tsv (bounded by
@@ -378,10 +398,10 @@ Zeus(Function const& f, double lower, double upper, std::size_t N, int MAX_ITER,
     if (gradient == Gradient::Adjoint)
       return detail::run<Function, D, Gradient::Adjoint>(f, lower, upper, N, MAX_ITER, PSO_ITER,
                                                          requiredConverged, fun_name, tolerance,
-                                                         seed, parallel, tr, run);
+                                                         seed, parallel, tr, run, kind);
   return detail::run<Function, D, Gradient::Dual>(f, lower, upper, N, MAX_ITER, PSO_ITER,
                                                   requiredConverged, fun_name, tolerance, seed,
-                                                  parallel, tr, run);
+                                                  parallel, tr, run, kind);
 }
 
 } // namespace zeus

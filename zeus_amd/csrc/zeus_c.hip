@@ -253,6 +253,8 @@ validate(const zeus_problem& p)
       break;
     default: return ZEUS_E_ARG;
   }
+  if (p.prng != ZEUS_PRNG_XORWOW && p.prng != ZEUS_PRNG_PHILOX && p.prng != ZEUS_PRNG_SOBOL)
+    return ZEUS_E_ARG;
   if (const zeus_trajectory* t = p.trajectory) {
     if (t->steps < 0 || t->starts < 0)
       return ZEUS_E_ARG;
@@ -351,6 +353,11 @@ struct ClassPlan {
   int ctl0 = 0;          // first control block of the class
   bool traced = false;   // some problem records a trajectory
   bool metrics = false;  // per-start timing requested (ZEUS_FLAG_METRICS)
+  bool rng = false;      // some problem draws XORWOW or SOBOL (prng.hpp)
+  bool gen_starts = false; // ... and some of those draws its BFGS starts
+  uint32_t* rng_state = nullptr; // XORWOW states, 6 words per start
+  double* draws = nullptr;       // one PSO iteration's uniforms, 2D x total
+  double* x0gen = nullptr;       // generated BFGS starts, total x D
   // host
   std::vector<ProblemDesc> desc;
   std::vector<int32_t> block_problem;
@@ -390,6 +397,11 @@ carve_class(Carver& cv, ClassPlan& c)
   c.aux = cv.take<double>(c.ops->objective == obj::kGaussian ? np * dev::gaussian_aux_doubles(D)
                                                               : 1);
   c.h_slab = cv.take<double>(size_t(std::max<int64_t>(c.slab, 1)));
+  if (c.rng) {
+    c.rng_state = cv.take<uint32_t>(6 * n);
+    c.draws = cv.take<double>(pso ? 2 * n * D : 1);
+    c.x0gen = cv.take<double>(c.gen_starts ? n * D : 1);
+  }
   if (c.metrics) {
     c.out.ad_ticks = cv.take<uint64_t>(n);
     c.out.bfgs_ticks = cv.take<uint64_t>(n);
@@ -405,8 +417,15 @@ carve_class(Carver& cv, ClassPlan& c)
 PsoArgs
 pso_args(const ClassPlan& c, Control* ctl)
 {
-  return PsoArgs{c.d_desc, int32_t(c.probs.size()), 0, c.pso_blocks, ctl, c.X, c.V, c.pbx,
-                 c.pbv, c.gbx, c.part_val, c.part_idx, c.part_x, c.gstate, c.d_block_problem};
+  return PsoArgs{c.d_desc,   int32_t(c.probs.size()),
+                 0,          c.pso_blocks,
+                 ctl,        c.X,
+                 c.V,        c.pbx,
+                 c.pbv,      c.gbx,
+                 c.part_val, c.part_idx,
+                 c.part_x,   c.gstate,
+                 c.d_block_problem, c.rng ? c.draws : nullptr,
+                 c.total};
 }
 
 hipError_t
@@ -426,10 +445,24 @@ launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
     if ((e = hipGetLastError()) != hipSuccess)
       return e;
   }
+  const int rg = int((c.total + 255) / 256);
+  if (c.rng && c.total > 0) {
+    dev::rng_init_kernel<<<rg, 256, 0, st>>>(c.d_desc, np, c.total, c.rng_state);
+    if (c.gen_starts)
+      dev::rng_starts_kernel<<<rg, 256, 0, st>>>(c.d_desc, np, c.total, c.rng_state, c.x0gen);
+    if ((e = hipGetLastError()) != hipSuccess)
+      return e;
+  }
   if (c.max_pso_iter >= 0 && c.pso_blocks > 0) {
     PsoArgs pa = pso_args(c, ctl);
     for (int it = 0; it <= c.max_pso_iter; ++it) {
       pa.iter = it;
+      if (c.rng) {
+        dev::rng_pso_draws_kernel<<<rg, 256, 0, st>>>(c.d_desc, np, c.total, it, c.rng_state,
+                                                      c.draws);
+        if ((e = hipGetLastError()) != hipSuccess)
+          return e;
+      }
       if ((e = c.ops->pso(pa, c.pso_blocks, st)) != hipSuccess)
         return e;
     }
@@ -598,6 +631,11 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       d.params = p.params;
       d.n_params = p.n_params;
       d.x0 = p.x0;
+      d.prng = p.prng;
+      if (d.prng != ZEUS_PRNG_PHILOX) {
+        c.rng = true;
+        c.gen_starts = c.gen_starts || (d.pso_iter < 0 && d.x0 == nullptr);
+      }
       d.traj = traj_of(p);
       c.traced = c.traced || d.traj.fval != nullptr;
       c.desc.push_back(d);
@@ -677,11 +715,14 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   auto mirror = [&](const void* dev_ptr) {
     return pin + (static_cast<const char*>(dev_ptr) - static_cast<char*>(g_ds.arena));
   };
-  // problems that run PSO start their BFGS from pBestX
+  // problems that run PSO start their BFGS from pBestX; stateful generators
+  // without PSO start from the generated points
   for (auto& c : classes)
     for (auto& d : c.desc) {
       if (d.x0 == nullptr && d.pso_iter >= 0)
         d.x0 = c.pbx + d.start0 * d.dim;
+      else if (d.x0 == nullptr && d.prng != ZEUS_PRNG_PHILOX)
+        d.x0 = c.x0gen + d.start0 * d.dim;
       if (c.ops->objective == obj::kGaussian) {
         d.aux = c.aux + (&d - c.desc.data()) * dev::gaussian_aux_doubles(d.dim);
         d.aux_frag = d.aux + int64_t(d.dim) * d.dim;
@@ -779,6 +820,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       key.push_back(uint64_t(int64_t(c.max_pso_iter)));
       key.push_back(uint64_t(c.probs.size()));
       key.push_back(uint64_t(c.cfg.form) << 32 | uint32_t(c.cfg.threads));
+      key.push_back(uint64_t(c.rng) << 1 | uint64_t(c.gen_starts));
     }
     hipGraphExec_t exec = nullptr;
     for (const auto& ge : g_ds.graphs)
@@ -932,6 +974,18 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
 // ===========================================================================
 namespace {
 
+// Draws 0..n-1 of one start's stream, in order, by one lane.
+__global__ void
+stream_fill_kernel(double* out, int64_t n, int prng, uint32_t seed, uint32_t stream,
+                   uint64_t start)
+{
+  rng::Xorwow st = rng::xorwow_seed(seed, stream, start);
+  for (int64_t k = 0; k < n; ++k)
+    out[k] = prng == rng::kXorwow  ? rng::xorwow_uniform(st)
+             : prng == rng::kSobol ? rng::sobol_uniform(start, uint64_t(k))
+                                   : rng::uniform(seed, stream, start, uint64_t(k));
+}
+
 __global__ void
 argmin_kernel(const double* v, int64_t n, int64_t* out)
 {
@@ -991,9 +1045,11 @@ zeus_pso_run(const zeus_problem* prob, double* pbest_x, double* pbest_val, doubl
   d.pso_iter = prob->pso_iter;
   d.seed = prob->seed;
   d.stream = prob->stream;
+  d.prng = prob->prng;
   d.params = prob->params;
   d.n_params = prob->n_params;
   c.desc.push_back(d);
+  c.rng = d.prng != ZEUS_PRNG_PHILOX;
   c.pso_blocks = int((prob->n_starts + c.ops->pso_ppb - 1) / c.ops->pso_ppb);
   c.block_problem.assign(c.pso_blocks, 0);
   c.total = prob->n_starts;
@@ -1113,6 +1169,26 @@ zeus_philox_fill(double* out, int64_t n, uint32_t seed, uint32_t stream, uint64_
     return rc;
   hipStream_t st = stream_handle ? static_cast<hipStream_t>(stream_handle) : g_ds.stream;
   philox_fill_kernel<<<int((n + 255) / 256), 256, 0, st>>>(out, n, seed, stream, start, draw0);
+  hipError_t e = hipGetLastError();
+  const hipError_t e2 = hipStreamSynchronize(st);
+  return to_status(e != hipSuccess ? e : e2);
+}
+
+zeus_status_t
+zeus_stream_fill(double* out, int64_t n, int32_t prng, uint32_t seed, uint32_t stream,
+                 uint64_t start, void* stream_handle)
+{
+  if (n < 0 || (n > 0 && !out) ||
+      (prng != ZEUS_PRNG_XORWOW && prng != ZEUS_PRNG_PHILOX && prng != ZEUS_PRNG_SOBOL))
+    return ZEUS_E_ARG;
+  if (n == 0)
+    return ZEUS_OK;
+  std::lock_guard<std::mutex> lock(g_mu);
+  zeus_status_t rc = ensure_device();
+  if (rc != ZEUS_OK)
+    return rc;
+  hipStream_t st = stream_handle ? static_cast<hipStream_t>(stream_handle) : g_ds.stream;
+  stream_fill_kernel<<<1, 1, 0, st>>>(out, n, prng, seed, stream, start);
   hipError_t e = hipGetLastError();
   const hipError_t e2 = hipStreamSynchronize(st);
   return to_status(e != hipSuccess ? e : e2);
