@@ -49,6 +49,37 @@ cs_unit_range(const double* Cf, const double* xt, int mb, int li, int lk)
   return acc;
 }
 
+// The same product split in two steps, so that a caller can issue a unit's
+// fragment loads early (before the points are staged) and run the MFMA
+// chain later: CsFrag<D>::load, then CsFrag<D>::mma.
+template <int D>
+struct CsFrag {
+  using S = CsShape<D>;
+  static constexpr int N = S::KB - S::KB1 > S::KB1 ? S::KB - S::KB1 : S::KB1;
+  double av[N];
+
+  ZEUS_D ZEUS_FI void load(const double* Cf, int u, int li, int lk)
+  {
+    const int kb0 = (u & 1) ? S::KB1 : 0, n = (u & 1) ? S::KB - S::KB1 : S::KB1;
+    const double* frag = Cf + (int64_t(u >> 1) * S::KB << 6) + (li + 16 * lk) + (kb0 << 6);
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+      av[k] = k < n ? frag[k << 6] : 0.0;
+  }
+
+  ZEUS_D ZEUS_FI mfma_acc_t mma(const double* xt, int u, int li, int lk) const
+  {
+    const int kb0 = (u & 1) ? S::KB1 : 0, n = (u & 1) ? S::KB - S::KB1 : S::KB1;
+    mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+      if (k < n)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], xt[(4 * (kb0 + k) + lk) * S::XS + li],
+                                                   acc, 0, 0, 0);
+    return acc;
+  }
+};
+
 // Unit u: rows 16 (u / 2) .. + 15 of Cs, k half u % 2.  Lane (li, lk) of the
 // result holds G[16 (u / 2) + 4 q + lk][li] in component q (partial over
 // the unit's k half).

@@ -52,6 +52,19 @@ uniform(uint32_t seed, uint32_t stream, uint64_t start, uint64_t draw)
   return double(m + 1u) * 0x1p-53;
 }
 
+// Draws 2b and 2b + 1 of a start (both halves of counter block b).
+ZEUS_HD ZEUS_FI void
+uniform_pair(uint32_t seed, uint32_t stream, uint64_t start, uint64_t blk, double& u0,
+             double& u1)
+{
+  const U32x4 o = philox4x32_10(uint32_t(blk), uint32_t(blk >> 32), uint32_t(start),
+                                uint32_t(start >> 32), seed, stream);
+  const uint64_t m0 = ((uint64_t(o.v[0]) << 32) | o.v[1]) >> 11;
+  const uint64_t m1 = ((uint64_t(o.v[2]) << 32) | o.v[3]) >> 11;
+  u0 = double(m0 + 1u) * 0x1p-53;
+  u1 = double(m1 + 1u) * 0x1p-53;
+}
+
 // reference include/utils.cuh:353-358: lower + (upper + (-lower)) * u
 ZEUS_HD ZEUS_FI double
 uniform_in(uint32_t seed, uint32_t stream, uint64_t start, uint64_t draw,

@@ -105,30 +105,61 @@ block_argmin(double& v, int64_t& i, double* sv, int64_t* si)
 // Replaced only on strict improvement.  Writes gb[0..D) (LDS) and returns
 // the value; with `store` (the problem's first PSO workgroup) the result is
 // kept for the next fold.  All threads of the workgroup call.
+//
+// Two steps: pso_fold_read issues every load that does not depend on the
+// winner (this thread's share of the partials, the previous gBest value and,
+// when a thread per coordinate is available, its coordinates), so that a
+// caller can issue further independent loads before pso_fold_finish; only
+// the winner's coordinates are read after the reduction.
+struct FoldIn {
+  double bv;
+  int64_t bi;
+  double gpv; // previous gBest value (thread 0)
+  double gpx; // previous gBest coordinate threadIdx.x (NT >= D)
+};
+
+template <int NT, int PPB, int D>
+ZEUS_D ZEUS_FI FoldIn
+pso_fold_read(const PsoArgs& a, int pi, const ProblemDesc& P, int t_prev)
+{
+  const int q = t_prev & 1;
+  const int nblk = int((P.n_starts + PPB - 1) / PPB);
+  FoldIn f{__builtin_nan(""), int64_t(1) << 62, __builtin_huge_val(), 0.0};
+  for (int b = threadIdx.x; b < nblk; b += NT) {
+    const int64_t o = int64_t(q) * a.n_blocks + P.block0 + b;
+    const double v = a.part_val[o];
+    const int64_t i = a.part_idx[o];
+    if (better(v, i, f.bv, f.bi)) {
+      f.bv = v;
+      f.bi = i;
+    }
+  }
+  const double* gprev = a.gstate + int64_t(pi) * 2 * (D + 1) + (q ^ 1) * (D + 1);
+  if (threadIdx.x == 0 && t_prev > 0)
+    f.gpv = gprev[0];
+  if constexpr (NT >= D)
+    if (int(threadIdx.x) < D && t_prev > 0)
+      f.gpx = gprev[1 + threadIdx.x];
+  return f;
+}
+
 template <int NT, int PPB, int D>
 ZEUS_D double
-pso_fold(const PsoArgs& a, int pi, const ProblemDesc& P, int t_prev, double* gb, bool store)
+pso_fold_finish(const PsoArgs& a, int pi, const ProblemDesc& P, int t_prev, double* gb,
+                bool store, FoldIn f)
 {
+  double bv = f.bv;
+  int64_t bi = f.bi;
   __shared__ double sv[NT / kWave];
   __shared__ int64_t si[NT / kWave];
   __shared__ double s_val;
   __shared__ int s_src; // -1: keep the previous gBest, else winning workgroup
   const int q = t_prev & 1;
-  const int nblk = int((P.n_starts + PPB - 1) / PPB);
-  double bv = __builtin_nan("");
-  int64_t bi = int64_t(1) << 62;
-  for (int b = threadIdx.x; b < nblk; b += NT) {
-    const int64_t o = int64_t(q) * a.n_blocks + P.block0 + b;
-    if (better(a.part_val[o], a.part_idx[o], bv, bi)) {
-      bv = a.part_val[o];
-      bi = a.part_idx[o];
-    }
-  }
   block_argmin<NT>(bv, bi, sv, si);
   double* gs = a.gstate + int64_t(pi) * 2 * (D + 1);
   const double* gprev = gs + (q ^ 1) * (D + 1); // stored during iteration t_prev
   if (threadIdx.x == 0) {
-    const double pv = t_prev == 0 ? __builtin_huge_val() : gprev[0];
+    const double pv = f.gpv; // +inf before iteration 0
     s_src = -1;
     s_val = pv;
     if (bv < pv) {
@@ -137,10 +168,17 @@ pso_fold(const PsoArgs& a, int pi, const ProblemDesc& P, int t_prev, double* gb,
     }
   }
   __syncthreads();
-  const double* src = s_src >= 0 ? a.part_x + (int64_t(q) * a.n_blocks + P.block0 + s_src) * D
-                                 : gprev + 1;
-  for (int d = threadIdx.x; d < D; d += NT)
-    gb[d] = src[d];
+  if (s_src >= 0) {
+    const double* src = a.part_x + (int64_t(q) * a.n_blocks + P.block0 + s_src) * D;
+    for (int d = threadIdx.x; d < D; d += NT)
+      gb[d] = src[d];
+  } else if constexpr (NT >= D) {
+    if (int(threadIdx.x) < D)
+      gb[threadIdx.x] = f.gpx;
+  } else {
+    for (int d = threadIdx.x; d < D; d += NT)
+      gb[d] = gprev[1 + d];
+  }
   const double val = s_val;
   if (store) {
     double* gnew = gs + q * (D + 1);
@@ -151,6 +189,14 @@ pso_fold(const PsoArgs& a, int pi, const ProblemDesc& P, int t_prev, double* gb,
   }
   __syncthreads();
   return val;
+}
+
+template <int NT, int PPB, int D>
+ZEUS_D double
+pso_fold(const PsoArgs& a, int pi, const ProblemDesc& P, int t_prev, double* gb, bool store)
+{
+  return pso_fold_finish<NT, PPB, D>(a, pi, P, t_prev, gb, store,
+                                     pso_fold_read<NT, PPB, D>(a, pi, P, t_prev));
 }
 
 // Position/velocity of one coordinate: init (t = 0) or update (t >= 1).
@@ -165,6 +211,65 @@ pso_uniform(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d, int
                            : rng::uniform(P.seed, P.stream, uint64_t(s - P.start0),
                                           rng::pso_draw(D, a.iter, d, j));
   return lo + (hi + (-lo)) * u;
+}
+
+// Both uniforms (j = 0, 1) of coordinate d of start s in this iteration:
+// one Philox block, or two reads of the draw buffer.
+ZEUS_D ZEUS_FI void
+pso_uniforms(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d, double& u0,
+             double& u1)
+{
+  if (a.draws) {
+    u0 = a.draws[int64_t(2 * d) * a.draws_stride + s];
+    u1 = a.draws[int64_t(2 * d + 1) * a.draws_stride + s];
+  } else {
+    rng::uniform_pair(P.seed, P.stream, uint64_t(s - P.start0),
+                      rng::pso_draw(D, a.iter, d, 0) >> 1, u0, u1);
+  }
+}
+
+// The operands of one coordinate's update that do not depend on gBest
+// (position, velocity, pBest and both uniforms), loaded ahead of the fold.
+struct PsoCoordIn {
+  double x, v, pb, u0, u1;
+};
+
+ZEUS_D ZEUS_FI PsoCoordIn
+pso_coord_load(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d)
+{
+  PsoCoordIn c{0.0, 0.0, 0.0, 0.0, 0.0};
+  pso_uniforms(a, P, D, s, d, c.u0, c.u1);
+  if (a.iter > 0) {
+    const int64_t o = s * D + d;
+    c.x = a.X[o];
+    c.v = a.V[o];
+    c.pb = a.pbx[o];
+  }
+  return c;
+}
+
+// pso_coord from preloaded operands: the same arithmetic, same order.
+ZEUS_D ZEUS_FI double
+pso_coord_apply(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d,
+                const PsoCoordIn& c, const double* gb)
+{
+  const int64_t o = s * D + d;
+  if (a.iter == 0) {
+    const double vr = (P.upper - P.lower) * 0.1;
+    const double rx = P.lower + (P.upper + (-P.lower)) * c.u0;
+    const double rv = -vr + (vr + (-(-vr))) * c.u1;
+    a.X[o] = rx;
+    a.V[o] = rv;
+    a.pbx[o] = rx;
+    return rx;
+  }
+  const double r1 = 0.0 + (1.0 + (-0.0)) * c.u0;
+  const double r2 = 0.0 + (1.0 + (-0.0)) * c.u1;
+  const double xi = c.x, vi = c.v, pb = c.pb;
+  const double nv = kPsoW * vi + kPsoC1 * r1 * (pb - xi) + kPsoC2 * r2 * (gb[d] - xi);
+  a.V[o] = nv;
+  a.X[o] = xi + nv;
+  return xi + nv;
 }
 
 ZEUS_D ZEUS_FI double
@@ -344,22 +449,48 @@ pso_gaussian_mfma_kernel(PsoArgs a)
 
   for (int e = tid; e < S::KP * S::XS; e += kPsoTileThreads)
     xs[e] = 0.0;
+  // the fold's partials are read first; then the Cs fragments of this
+  // wavefront's first product unit and this thread's coordinate operands and
+  // draws (coalesced over d), none of which depends on gBest, so that their
+  // latency overlaps the rest of the fold (loads complete in issue order)
+  FoldIn fin{};
   if (a.iter > 0)
-    (void)pso_fold<kPsoTileThreads, kPsoTile, D>(a, pi, P, a.iter - 1, gb,
-                                                 int(blockIdx.x) == P.block0);
-  __syncthreads();
-  // update / init coordinates (coalesced over d) and stage the tile
+    fin = pso_fold_read<kPsoTileThreads, kPsoTile, D>(a, pi, P, a.iter - 1);
+  CsFrag<D> frag0;
+  if (w < S::U)
+    frag0.load(P.aux_frag, w, li, lk);
+  constexpr int kE = (kPsoTile * D + kPsoTileThreads - 1) / kPsoTileThreads;
+  PsoCoordIn cin[kE];
 #pragma unroll
-  for (int e = tid; e < kPsoTile * D; e += kPsoTileThreads) {
+  for (int k = 0; k < kE; ++k) {
+    const int e = tid + k * kPsoTileThreads;
     const int j = e / D, d = e % D;
-    const int64_t sl = base + j;
-    if (sl < P.n_starts)
-      xs[d * S::XS + j] = pso_coord(a, P, D, P.start0 + sl, d, gb);
+    if (e < kPsoTile * D && base + j < P.n_starts)
+      cin[k] = pso_coord_load(a, P, D, P.start0 + base + j, d);
+  }
+  double old_pbv = 0.0;
+  if (tid < kPsoTile && a.iter > 0 && base + tid < P.n_starts)
+    old_pbv = a.pbv[P.start0 + base + tid];
+  if (a.iter > 0)
+    (void)pso_fold_finish<kPsoTileThreads, kPsoTile, D>(a, pi, P, a.iter - 1, gb,
+                                                        int(blockIdx.x) == P.block0, fin);
+  __syncthreads();
+  // update / init coordinates and stage the tile
+#pragma unroll
+  for (int k = 0; k < kE; ++k) {
+    const int e = tid + k * kPsoTileThreads;
+    const int j = e / D, d = e % D;
+    if (e < kPsoTile * D && base + j < P.n_starts)
+      xs[d * S::XS + j] = pso_coord_apply(a, P, D, P.start0 + base + j, d, cin[k], gb);
   }
   __syncthreads();
 
+  static_assert(S::U <= 2 * NW, "at most two product units per wavefront");
+  CsFrag<D> frag1;
+  if (w + NW < S::U)
+    frag1.load(P.aux_frag, w + NW, li, lk); // in flight behind the first chain
   for (int u = w; u < S::U; u += NW) {
-    const mfma_acc_t acc = cs_unit<D>(P.aux_frag, xs, u, li, lk);
+    const mfma_acc_t acc = u == w ? frag0.mma(xs, u, li, lk) : frag1.mma(xs, u, li, lk);
     double fp = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -389,7 +520,7 @@ pso_gaussian_mfma_kernel(PsoArgs a)
         a.pbv[s] = fv;
         v = fv;
       } else {
-        v = a.pbv[s];
+        v = old_pbv;
         if (fv < v) {
           a.pbv[s] = fv;
           v = fv;
@@ -401,18 +532,15 @@ pso_gaussian_mfma_kernel(PsoArgs a)
     improved[tid] = imp;
   }
   __syncthreads();
-  if (tid == 0) { // best particle of the workgroup (NaN loses, ties to the lowest index)
-    double bv = __builtin_nan("");
-    int64_t bi = int64_t(1) << 62;
-    int bj = 0;
-    for (int j = 0; j < kPsoTile; ++j)
-      if (base + j < P.n_starts && better(pbv[j], base + j, bv, bi)) {
-        bv = pbv[j];
-        bi = base + j;
-        bj = j;
-      }
-    best_j = bj;
-    pso_publish_best(a, bv, bi);
+  if (w == 0) { // best particle of the workgroup (NaN loses, ties to the lowest index)
+    const bool ok = lane < kPsoTile && base + lane < P.n_starts;
+    double bv = ok ? pbv[lane] : __builtin_nan("");
+    int64_t bi = ok ? base + lane : int64_t(1) << 62;
+    wave_argmin(bv, bi);
+    if (lane == 0) {
+      best_j = int(bi - base);
+      pso_publish_best(a, bv, bi);
+    }
   }
   __syncthreads();
   // pBest rows of improved particles, and the best particle's coordinates:
