@@ -58,10 +58,11 @@ typedef int32_t zeus_status_t;
 #define ZEUS_OBJ_SPHERE 7     /* params: none                              */
 
 /* Generators (reference PRNGType, include/traits.hpp:67-71), one stream of
-   uniforms in (0, 1] per start: XORWOW (Marsaglia 2003, a state per start
-   seeded from (seed, stream, start)), PHILOX (counter-based Philox4x32-10
-   keyed by (seed, stream)), SOBOL (dimension 1 of Sobol's sequence read from
-   index start on; seed unused).  None reproduces cuRAND's streams. */
+   uniforms in (0, 1]: XORWOW (Marsaglia 2003; a state per (start,
+   coordinate) seeded from (seed, stream, start, coordinate)), PHILOX
+   (counter-based Philox4x32-10 keyed by (seed, stream), one stream per
+   start), SOBOL (dimension 1 of Sobol's sequence read from index start on;
+   seed unused).  None reproduces cuRAND's streams. */
 #define ZEUS_PRNG_XORWOW 0
 #define ZEUS_PRNG_PHILOX 1
 #define ZEUS_PRNG_SOBOL 2
@@ -233,10 +234,13 @@ zeus_status_t zeus_philox_fill(double* out, int64_t n, uint32_t seed,
                                uint32_t stream, uint64_t start,
                                uint64_t draw0, void* stream_handle);
 
-/* Device fill of draws 0..n-1 of one start's stream under generator prng
-   (ZEUS_PRNG_*), in the order the optimizer consumes them. */
+/* Device fill of draws 0..n-1 of one stream under generator prng
+   (ZEUS_PRNG_*), in the order the optimizer consumes them: XORWOW, the
+   stream of (start, coord); PHILOX and SOBOL, the draw indices of start
+   (coord unused). */
 zeus_status_t zeus_stream_fill(double* out, int64_t n, int32_t prng, uint32_t seed,
-                               uint32_t stream, uint64_t start, void* stream_handle);
+                               uint32_t stream, uint64_t start, uint32_t coord,
+                               void* stream_handle);
 
 /* Device memory (replaces CudaBuffer's cudaMalloc/cudaMemcpy,
    include/cuda_buffer.hpp:14-209). */

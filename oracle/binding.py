@@ -111,7 +111,7 @@ def lib():
         L.orc_pso.argtypes = [C.POINTER(Problem), _dp, _dp, _dp, _dp]
         L.orc_pso_step.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
                                    C.c_double, _dp, _dp, _dp, _dp, _dp]
-        L.orc_xorwow_seed.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64,
+        L.orc_xorwow_seed.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
                                       C.POINTER(C.c_uint32)]
         L.orc_xorwow_next.restype = C.c_uint32
         L.orc_xorwow_next.argtypes = [C.POINTER(C.c_uint32)]
@@ -119,7 +119,7 @@ def lib():
         L.orc_sobol_uniform.argtypes = [C.c_uint64, C.c_uint64]
         L.orc_stream_uniforms.restype = C.c_int
         L.orc_stream_uniforms.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint64,
-                                          C.c_int64, _dp]
+                                          C.c_uint32, C.c_int64, _dp]
         L.orc_argmin.restype = C.c_int64
         L.orc_argmin.argtypes = [_dp, C.c_int64]
         _lib = L
@@ -256,9 +256,9 @@ def _problem(objective, dim, lower, upper, n_starts, required, max_iter, pso_ite
                    _ptr(params), 0 if params is None else params.size, _ptr(x0), prng)
 
 
-def xorwow_seed(seed, stream, start):
+def xorwow_seed(seed, stream, start, coord=0):
     st = (C.c_uint32 * 6)()
-    lib().orc_xorwow_seed(seed & 0xFFFFFFFF, stream & 0xFFFFFFFF, start, st)
+    lib().orc_xorwow_seed(seed & 0xFFFFFFFF, stream & 0xFFFFFFFF, start, coord, st)
     return list(st)
 
 
@@ -274,10 +274,10 @@ def sobol_uniform(start, draw):
     return lib().orc_sobol_uniform(start, draw)
 
 
-def stream_uniforms(prng, seed, stream, start, n):
+def stream_uniforms(prng, seed, stream, start, n, coord=0):
     out = np.zeros(n)
-    rc = lib().orc_stream_uniforms(prng, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF, start, n,
-                                   _ptr(out))
+    rc = lib().orc_stream_uniforms(prng, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF, start, coord,
+                                   n, _ptr(out))
     if rc != 0:
         raise ValueError("bad generator arguments")
     return out

@@ -69,18 +69,22 @@ orc_uniform(uint32_t seed, uint32_t stream, uint64_t start, uint64_t draw)
 }
 
 /* ---- selectable generators (include/traits.hpp:67-71) ----------------
-   XORWOW: Marsaglia, "Xorshift RNGs", J. Stat. Softw. 8(14), 2003; the
-   state of a start is seeded from Philox counter blocks 2^64-1 and 2^64-2
-   (never used by draws); a uniform takes two outputs, (m + 1) 2^-53 with
-   m = top 32 bits of the first and top 21 of the second.
+   XORWOW: Marsaglia, "Xorshift RNGs", J. Stat. Softw. 8(14), 2003; one
+   state per (start, coordinate), seeded from Philox counter blocks
+   (2c, 2^32-1) and (2c+1, 2^32-1) of the start (never used by draws), giving
+   that coordinate's draws in order (PSO init position and velocity, then r1
+   and r2 of each iteration, or one BFGS random-start draw); a uniform takes
+   two outputs, (m + 1) 2^-53 with m = top 32 bits of the first and top 21
+   of the second.
    SOBOL: dimension 1 (direction numbers 2^(31-j)): point n is
-   bitreverse32(n ^ (n >> 1)); start i reads from point i on; (x + 1) 2^-32. */
+   bitreverse32(n ^ (n >> 1)); draw k of start i is point i + k; (x + 1) 2^-32. */
 void
-orc_xorwow_seed(uint32_t seed, uint32_t stream, uint64_t start, uint32_t st[6])
+orc_xorwow_seed(uint32_t seed, uint32_t stream, uint64_t start, uint32_t coord,
+                uint32_t st[6])
 {
   uint32_t key[2] = {seed, stream};
-  uint32_t ca[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, (uint32_t)start, (uint32_t)(start >> 32)};
-  uint32_t cb[4] = {0xFFFFFFFEu, 0xFFFFFFFFu, (uint32_t)start, (uint32_t)(start >> 32)};
+  uint32_t ca[4] = {2u * coord, 0xFFFFFFFFu, (uint32_t)start, (uint32_t)(start >> 32)};
+  uint32_t cb[4] = {2u * coord + 1u, 0xFFFFFFFFu, (uint32_t)start, (uint32_t)(start >> 32)};
   uint32_t a[4], b[4];
   orc_philox4x32_10(ca, key, a);
   orc_philox4x32_10(cb, key, b);
@@ -126,52 +130,51 @@ orc_sobol_uniform(uint64_t start, uint64_t draw)
   return ((double)r + 1.0) * 0x1p-32;
 }
 
-/* One start's stream: draws are requested in increasing index order. */
-typedef struct {
-  int kind;
-  uint32_t seed, stream;
-  uint64_t start, next;
-  uint32_t xs[6];
-} orc_stream;
-
-static void
-stream_init(orc_stream* r, const orc_problem* P, uint64_t start)
+/* Draw `draw` of start `start` for the stateless generators. */
+static double
+index_uniform(const orc_problem* P, uint64_t start, uint64_t draw)
 {
-  r->kind = P->prng;
-  r->seed = P->seed;
-  r->stream = P->stream;
-  r->start = start;
-  r->next = 0;
-  if (r->kind == ORC_PRNG_XORWOW)
-    orc_xorwow_seed(P->seed, P->stream, start, r->xs);
+  return P->prng == ORC_PRNG_SOBOL ? orc_sobol_uniform(start, draw)
+                                   : orc_uniform(P->seed, P->stream, start, draw);
 }
 
-static double
-stream_uniform(orc_stream* r, uint64_t draw)
+/* Uniforms u0, u1 of coordinate d of start i at draw index k (= 2Dt + 2d):
+   PHILOX / SOBOL by index, XORWOW from the (start, coordinate) state. */
+static void
+pso_pair(const orc_problem* P, uint64_t i, uint64_t k, uint32_t* xs, double* u0, double* u1)
 {
-  if (r->kind == ORC_PRNG_PHILOX)
-    return orc_uniform(r->seed, r->stream, r->start, draw);
-  if (r->kind == ORC_PRNG_SOBOL)
-    return orc_sobol_uniform(r->start, draw);
-  while (r->next < draw) {
-    (void)xorwow_uniform(r->xs);
-    ++r->next;
+  if (P->prng == ORC_PRNG_XORWOW) {
+    *u0 = xorwow_uniform(xs);
+    *u1 = xorwow_uniform(xs);
+  } else {
+    *u0 = index_uniform(P, i, k);
+    *u1 = index_uniform(P, i, k + 1u);
   }
-  ++r->next;
-  return xorwow_uniform(r->xs);
 }
 
 /* include/utils.cuh:353-358: lower + (upper + (-lower)) * u */
 static double
-uniform_in(orc_stream* r, uint64_t draw, double lo, double hi)
+to_range(double u, double lo, double hi)
 {
-  double u = stream_uniform(r, draw);
   return lo + (hi + (-lo)) * u;
 }
 
+/* Coordinate d of a BFGS random start of start i (draw index d, or the
+   first draw of the (start, coordinate) XORWOW stream). */
+static double
+start_uniform_in(const orc_problem* P, uint64_t i, int d)
+{
+  if (P->prng == ORC_PRNG_XORWOW) {
+    uint32_t st[6];
+    orc_xorwow_seed(P->seed, P->stream, i, (uint32_t)d, st);
+    return to_range(xorwow_uniform(st), P->lower, P->upper);
+  }
+  return to_range(index_uniform(P, i, (uint64_t)d), P->lower, P->upper);
+}
+
 int
-orc_stream_uniforms(int prng, uint32_t seed, uint32_t stream, uint64_t start, int64_t n,
-                    double* out)
+orc_stream_uniforms(int prng, uint32_t seed, uint32_t stream, uint64_t start, uint32_t coord,
+                    int64_t n, double* out)
 {
   if (prng < 0 || prng > 2 || n < 0 || (n > 0 && !out))
     return -1;
@@ -180,10 +183,10 @@ orc_stream_uniforms(int prng, uint32_t seed, uint32_t stream, uint64_t start, in
   P.prng = prng;
   P.seed = seed;
   P.stream = stream;
-  orc_stream r;
-  stream_init(&r, &P, start);
+  uint32_t st[6];
+  orc_xorwow_seed(seed, stream, start, coord, st);
   for (int64_t k = 0; k < n; ++k)
-    out[k] = stream_uniform(&r, (uint64_t)k);
+    out[k] = prng == ORC_PRNG_XORWOW ? xorwow_uniform(st) : index_uniform(&P, start, (uint64_t)k);
   return 0;
 }
 
@@ -973,8 +976,8 @@ pso_run(const orc_problem* P, int nt, double* pbx, double* pbv, double* gbx,
     free(V);
     return -2;
   }
-  orc_stream* rs = (orc_stream*)malloc(sizeof(orc_stream) * (size_t)(N > 0 ? N : 1));
-  if (!rs) {
+  uint32_t* xs = (uint32_t*)malloc(sizeof(uint32_t) * 6 * (size_t)(N > 0 ? N : 1) * (size_t)D);
+  if (!xs) {
     free(X);
     free(V);
     return -2;
@@ -985,11 +988,14 @@ pso_run(const orc_problem* P, int nt, double* pbx, double* pbv, double* gbx,
 
 #pragma omp parallel for schedule(static) num_threads(nt)
   for (int64_t i = 0; i < N; ++i) {
-    stream_init(&rs[i], P, (uint64_t)i);
     for (int d = 0; d < D; ++d) {
       size_t o = (size_t)i * D + d;
-      double rx = uniform_in(&rs[i], 2u * d, lo, hi);
-      double rv = uniform_in(&rs[i], 2u * d + 1u, -vr, vr);
+      uint32_t* st = xs + 6 * o;
+      orc_xorwow_seed(P->seed, P->stream, (uint64_t)i, (uint32_t)d, st);
+      double u0, u1;
+      pso_pair(P, (uint64_t)i, 2u * (uint64_t)d, st, &u0, &u1);
+      double rx = to_range(u0, lo, hi);
+      double rv = to_range(u1, -vr, vr);
       X[o] = rx;
       V[o] = rv;
       pbx[o] = rx;
@@ -1004,8 +1010,10 @@ pso_run(const orc_problem* P, int nt, double* pbx, double* pbv, double* gbx,
     for (int64_t i = 0; i < N; ++i) {
       for (int d = 0; d < D; ++d) {
         size_t o = (size_t)i * D + d;
-        double r1 = uniform_in(&rs[i], base + 2u * d, 0.0, 1.0);
-        double r2 = uniform_in(&rs[i], base + 2u * d + 1u, 0.0, 1.0);
+        double u0, u1;
+        pso_pair(P, (uint64_t)i, base + 2u * (uint64_t)d, xs + 6 * o, &u0, &u1);
+        double r1 = to_range(u0, 0.0, 1.0);
+        double r2 = to_range(u1, 0.0, 1.0);
         double xi = X[o], vi = V[o], pb = pbx[o], gb = gbx[d];
         double nv = w * vi + c1 * r1 * (pb - xi) + c2 * r2 * (gb - xi);
         V[o] = nv;
@@ -1019,7 +1027,7 @@ pso_run(const orc_problem* P, int nt, double* pbx, double* pbv, double* gbx,
     }
     pso_gbest(pbv, pbx, N, D, gbv, gbx);
   }
-  free(rs);
+  free(xs);
   free(X);
   free(V);
   return 0;
@@ -1081,11 +1089,8 @@ run_start(const orc_problem* P, const orc_options* O, int64_t s,
   for (int i = 0; i < D; ++i)
     for (int j = 0; j < D; ++j)
       H[i * D + j] = (i == j) ? 1.0 : 0.0;
-  orc_stream rs;
-  stream_init(&rs, P, (uint64_t)s);
   for (int d = 0; d < D; ++d) {
-    x[d] = starts ? starts[(size_t)s * D + d]
-                  : uniform_in(&rs, (uint64_t)d, P->lower, P->upper);
+    x[d] = starts ? starts[(size_t)s * D + d] : start_uniform_in(P, (uint64_t)s, d);
     g[d] = 0.0;
   }
   double f0 = orc_value(&f, x);

@@ -120,13 +120,15 @@ typedef struct {
   int prng;         /* ORC_PRNG_*                                        */
 } orc_problem;
 
-/* Generators: XORWOW state seeding and step, SOBOL dimension-1 draw, and n
-   consecutive uniforms of one start's stream (returns 0, -1 on bad args). */
-void orc_xorwow_seed(uint32_t seed, uint32_t stream, uint64_t start, uint32_t st[6]);
+/* Generators: XORWOW state seeding (per start and coordinate) and step,
+   SOBOL dimension-1 draw, and n consecutive uniforms of one stream (XORWOW:
+   of (start, coord); PHILOX / SOBOL: of start) -- returns 0, -1 on bad args. */
+void orc_xorwow_seed(uint32_t seed, uint32_t stream, uint64_t start, uint32_t coord,
+                     uint32_t st[6]);
 uint32_t orc_xorwow_next(uint32_t st[6]);
 double orc_sobol_uniform(uint64_t start, uint64_t draw);
-int orc_stream_uniforms(int prng, uint32_t seed, uint32_t stream, uint64_t start, int64_t n,
-                        double* out);
+int orc_stream_uniforms(int prng, uint32_t seed, uint32_t stream, uint64_t start,
+                        uint32_t coord, int64_t n, double* out);
 
 /* Trajectory recording (include/zeus.cuh:62-103, bfgs_sequential.cuh:
    100-109, bfgs_parallel.cuh:155-184, bfgs_common.cuh:108-238): for step

@@ -276,9 +276,9 @@ def test_batch_equals_individual_calls():
 def test_dijet_fits_thread_form(orc):
     """Binned Poisson fits.  The NLL sums ~1e6-event terms, so its gradient
     has a roundoff floor far above 1e-8: starts surrender at MAX_ITER with x
-    determined to ~1e-6 relative (libm vs device transcendentals move the
+    determined to ~1e-5 relative (libm vs device transcendentals move the
     last bits, the flat minimum amplifies them); f is compared to 1e-8 and
-    the coordinates to 1e-5 relative."""
+    the coordinates to 5e-5 relative."""
     n_fit, N, iters = 4, 128, 400
     for e in range(n_fit):
         P = z.make_dijet(42, e)
@@ -287,7 +287,7 @@ def test_dijet_fits_thread_form(orc):
         o = orc.run(orc.LOGLIK, 4, 0, 10, N, iters, 10, N + 1, 1e-8, 42, stream=e, params=P)
         assert r.status == o.best["status"]
         assert close_f(r.fval, o.best["fval"])
-        assert np.all(np.abs(r.coordinates - o.best_x) <= 1e-5 * np.maximum(1, np.abs(o.best_x)))
+        assert np.all(np.abs(r.coordinates - o.best_x) <= 5e-5 * np.maximum(1, np.abs(o.best_x)))
         assert (r.per_start["status"] == o.status).mean() >= 0.95
 
 

@@ -279,9 +279,11 @@ def test_pso_init_and_invariants(orc):
         assert all(pbv0[i] == f.value(pbx0[i]) for i in range(37))
         assert gbv0 == pbv0.min() and np.array_equal(gbx0, pbx0[np.argmin(pbv0)])
         assert pbx0.min() > -5 and pbx0.max() <= 5
-        # starting draws are draw 2 (position of coordinate 1) of start 3's
-        # stream, in (lo, hi], for every generator
-        assert pbx0[3, 1] == -5 + (5 + 5) * orc.stream_uniforms(orc.PRNG_XORWOW, 42, 0, 3, 3)[2]
+        # the starting position of coordinate 1 of start 3, in (lo, hi]: the
+        # first draw of its XORWOW stream, draw 2 of the start for PHILOX
+        # and SOBOL
+        assert pbx0[3, 1] == -5 + (5 + 5) * orc.stream_uniforms(orc.PRNG_XORWOW, 42, 0, 3, 1,
+                                                                1)[0]
         px = orc.pso(obj, D, -5, 5, 37, 0, 42, prng=orc.PRNG_PHILOX)[0]
         assert px[3, 1] == -5 + (5 + 5) * orc.uniform(42, 0, 3, 2)
         ps = orc.pso(obj, D, -5, 5, 37, 0, 42, prng=orc.PRNG_SOBOL)[0]

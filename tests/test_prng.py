@@ -1,7 +1,8 @@
 """The selectable generators (reference PRNGType, include/traits.hpp:67-71):
-XORWOW against Marsaglia's published recurrence, its seeding against the
-Philox blocks it is defined from, SOBOL against the closed form of its
-first dimension; then the GPU streams and whole runs against the oracle."""
+XORWOW against Marsaglia's published recurrence, its per-(start,
+coordinate) seeding against the Philox blocks it is defined from, SOBOL
+against the closed form of its first dimension; then the GPU streams and
+whole runs against the oracle."""
 import numpy as np
 import pytest
 
@@ -33,14 +34,15 @@ def test_xorwow_is_marsaglias_recurrence(orc):
 
 
 def test_xorwow_seeding_and_conversion(orc):
-    for seed, stream, start in ((42, 0, 0), (7, 3, 12345), (0xDEADBEEF, 9, 2**40 + 3)):
-        st = orc.xorwow_seed(seed, stream, start)
+    for seed, stream, start, coord in ((42, 0, 0, 0), (7, 3, 12345, 149),
+                                       (0xDEADBEEF, 9, 2**40 + 3, 320)):
+        st = orc.xorwow_seed(seed, stream, start, coord)
         lo, hi = start & M32, start >> 32
-        a = orc.philox([M32, M32, lo, hi], [seed, stream])
-        b = orc.philox([M32 - 1, M32, lo, hi], [seed, stream])
+        a = orc.philox([2 * coord, M32, lo, hi], [seed, stream])
+        b = orc.philox([2 * coord + 1, M32, lo, hi], [seed, stream])
         assert st == a + b[:2]
         # a uniform is two outputs: top 32 + next 21 bits, (m + 1) 2^-53
-        u = orc.stream_uniforms(orc.PRNG_XORWOW, seed, stream, start, 50)
+        u = orc.stream_uniforms(orc.PRNG_XORWOW, seed, stream, start, 50, coord)
         raw = _marsaglia_xorwow(st, 100)
         for k in range(50):
             m = (raw[2 * k] << 21) | (raw[2 * k + 1] >> 11)
@@ -48,17 +50,18 @@ def test_xorwow_seeding_and_conversion(orc):
 
 
 def test_xorwow_statistics(orc):
-    u = np.concatenate([orc.stream_uniforms(orc.PRNG_XORWOW, 42, 0, s, 2000)
-                        for s in range(100)])
+    u = np.concatenate([orc.stream_uniforms(orc.PRNG_XORWOW, 42, 0, s, 2000, c)
+                        for s in range(20) for c in range(5)])
     assert u.min() > 0.0 and u.max() <= 1.0
     assert abs(u.mean() - 0.5) < 0.005
     h = np.histogram(u, bins=20, range=(0, 1))[0]
     chi2 = ((h - u.size / 20) ** 2 / (u.size / 20)).sum()
     assert chi2 < 45.0  # 19 degrees of freedom, p ~ 1e-3
-    # distinct starts get distinct streams
-    a = orc.stream_uniforms(orc.PRNG_XORWOW, 42, 0, 5, 100)
-    b = orc.stream_uniforms(orc.PRNG_XORWOW, 42, 0, 6, 100)
-    assert not np.any(a == b)
+    # distinct starts and coordinates get distinct streams
+    a = orc.stream_uniforms(orc.PRNG_XORWOW, 42, 0, 5, 100, 0)
+    b = orc.stream_uniforms(orc.PRNG_XORWOW, 42, 0, 6, 100, 0)
+    c = orc.stream_uniforms(orc.PRNG_XORWOW, 42, 0, 5, 100, 1)
+    assert not np.any(a == b) and not np.any(a == c)
 
 
 def test_sobol_first_dimension(orc):
@@ -99,9 +102,9 @@ GENS = [z.PRNGType.XORWOW, z.PRNGType.PHILOX, z.PRNGType.SOBOL]
 @pytest.mark.gpu
 def test_device_streams_match_oracle(orc):
     for g in GENS:
-        for start in (0, 77, 2**33 + 5):
-            got = z.stream_uniforms(g, 42, 5, start, 3000)
-            want = orc.stream_uniforms(int(g), 42, 5, start, 3000)
+        for start, coord in ((0, 0), (77, 3), (2**33 + 5, 149)):
+            got = z.stream_uniforms(g, 42, 5, start, 3000, coord)
+            want = orc.stream_uniforms(int(g), 42, 5, start, 3000, coord)
             assert np.array_equal(got, want), (g, start)
 
 

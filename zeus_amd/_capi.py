@@ -79,7 +79,7 @@ SIGNATURES = {
                                             C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                             C.POINTER(Result), _dp]),
     "zeus_stream_fill": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, C.c_uint32, C.c_uint32,
-                                     C.c_uint64, C.c_void_p]),
+                                     C.c_uint64, C.c_uint32, C.c_void_p]),
     "zeus_write_trajectory": (C.c_int32, [C.c_char_p, _dp, _dp, _dp, C.c_void_p, _dp,
                                           C.c_int32, C.c_int32, C.c_int32]),
     "zeus_pso_run": (C.c_int32, [C.POINTER(Problem), C.c_void_p, C.c_void_p, C.c_void_p,

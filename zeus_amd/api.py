@@ -392,12 +392,13 @@ def Zeus(f: Objective, lower: float, upper: float, N: int, MAX_ITER: int, PSO_IT
     return res
 
 
-def stream_uniforms(prng, seed: int, stream: int, start: int, n: int) -> np.ndarray:
-    """Draws 0..n-1 of one start's stream, generated on the device
-    (zeus_stream_fill)."""
+def stream_uniforms(prng, seed: int, stream: int, start: int, n: int,
+                    coord: int = 0) -> np.ndarray:
+    """Draws 0..n-1 of one stream, generated on the device (zeus_stream_fill):
+    XORWOW, that of (start, coord); PHILOX / SOBOL, that of start."""
     out = DeviceArray(nbytes=8 * n)
     check(lib().zeus_stream_fill(out.ptr, int(n), int(prng), int(seed) & 0xFFFFFFFF,
-                                 int(stream) & 0xFFFFFFFF, int(start), None),
+                                 int(stream) & 0xFFFFFFFF, int(start), int(coord), None),
           "zeus_stream_fill")
     return out.to_host()
 

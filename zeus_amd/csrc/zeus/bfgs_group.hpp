@@ -20,6 +20,7 @@
 #include "block.hpp"
 #include "lane_ops.hpp"
 #include "philox.hpp"
+#include "prng.hpp"
 
 namespace zeus::dev {
 
@@ -488,8 +489,7 @@ group_bfgs(const F& f, GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval&
     else if (P.x0)
       L.x[d] = P.x0[sl * D + d];
     else
-      L.x[d] = rng::uniform_in(P.seed, P.stream, uint64_t(sl), uint64_t(d), P.lower,
-                               P.upper);
+      L.x[d] = start_uniform_in(P, uint64_t(sl), d);
   }
   __syncthreads();
   double bestVal = ev.value(f, L.x, blk);

@@ -16,6 +16,7 @@
 #include "batch.hpp"
 #include "lane_ops.hpp"
 #include "philox.hpp"
+#include "prng.hpp"
 
 namespace zeus::dev {
 
@@ -78,8 +79,7 @@ thread_bfgs(const F& f, const ProblemDesc& P, Control* ctl, int64_t s,
     else if (P.x0)
       x[d] = P.x0[sl * D + d];
     else
-      x[d] = rng::uniform_in(P.seed, P.stream, uint64_t(sl), uint64_t(d), P.lower,
-                             P.upper);
+      x[d] = start_uniform_in(P, uint64_t(sl), d);
     g[d] = 0.0;
   }
 

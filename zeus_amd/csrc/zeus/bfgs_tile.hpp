@@ -31,6 +31,7 @@
 #include "block.hpp"
 #include "cs_product.hpp"
 #include "philox.hpp"
+#include "prng.hpp"
 
 namespace zeus::dev {
 
@@ -401,7 +402,7 @@ bfgs_tile_kernel(TileArgs a)
         for (int d = lane; d < D; d += kWave) {
           const double v =
             P.x0 ? P.x0[sl * D + d]
-                 : rng::uniform_in(P.seed, P.stream, uint64_t(sl), uint64_t(d), P.lower, P.upper);
+                 : start_uniform_in(P, uint64_t(sl), d);
           X[d] = v;
           L.xt[d * S::XS + j] = v;
         }

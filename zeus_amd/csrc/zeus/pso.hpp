@@ -26,6 +26,7 @@
 #include "cs_product.hpp"
 #include "lane_ops.hpp"
 #include "philox.hpp"
+#include "prng.hpp"
 
 namespace zeus::dev {
 
@@ -45,10 +46,10 @@ struct PsoArgs {
   double* part_x;      // [2][n_blocks][D]: its coordinates
   double* gstate;      // n_probs x [2][1 + D]: gBest (value, x) by iteration parity
   const int32_t* block_problem; // workgroup -> problem
-  // XORWOW / SOBOL streams: this iteration's uniforms, draws[k * draws_stride
-  // + s] for k = 2d + j (prng.hpp); nullptr: every problem draws Philox
-  const double* draws;
-  int64_t draws_stride;
+  // XORWOW states of the class's (start, coordinate) pairs (prng.hpp), word
+  // k at xw[k * xw_stride + s * D + d]; nullptr when no problem uses XORWOW
+  uint32_t* xw;
+  int64_t xw_stride;
 };
 
 // (value, index) order for argmin: NaN loses to any number, ties and the
@@ -201,30 +202,28 @@ pso_fold(const PsoArgs& a, int pi, const ProblemDesc& P, int t_prev, double* gb,
 
 // Position/velocity of one coordinate: init (t = 0) or update (t >= 1).
 // Returns the new position.
-// Uniform j of coordinate d of start s in this iteration, mapped to
-// [lo, hi] as reference include/utils.cuh:353-358 does.
-ZEUS_D ZEUS_FI double
-pso_uniform(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d, int j, double lo,
-            double hi)
-{
-  const double u = a.draws ? a.draws[int64_t(2 * d + j) * a.draws_stride + s]
-                           : rng::uniform(P.seed, P.stream, uint64_t(s - P.start0),
-                                          rng::pso_draw(D, a.iter, d, j));
-  return lo + (hi + (-lo)) * u;
-}
-
-// Both uniforms (j = 0, 1) of coordinate d of start s in this iteration:
-// one Philox block, or two reads of the draw buffer.
+// Both uniforms (j = 0, 1) of coordinate d of start s in this iteration
+// (draws 2Dt + 2d + j): one Philox block, two Sobol points, or the next two
+// draws of the (start, coordinate) XORWOW state, which is advanced.
 ZEUS_D ZEUS_FI void
 pso_uniforms(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d, double& u0,
              double& u1)
 {
-  if (a.draws) {
-    u0 = a.draws[int64_t(2 * d) * a.draws_stride + s];
-    u1 = a.draws[int64_t(2 * d + 1) * a.draws_stride + s];
+  const uint64_t sl = uint64_t(s - P.start0);
+  const uint64_t k = rng::pso_draw(D, a.iter, d, 0);
+  if (P.prng == rng::kXorwow) {
+    const int64_t o = s * D + d;
+    rng::Xorwow st = a.iter == 0 ? rng::xorwow_seed(P.seed, P.stream, sl, uint32_t(d))
+                                 : xw_load(a.xw, a.xw_stride, o);
+    u0 = rng::xorwow_uniform(st);
+    u1 = rng::xorwow_uniform(st);
+    if (a.iter < P.pso_iter)
+      xw_store(a.xw, a.xw_stride, o, st);
+  } else if (P.prng == rng::kSobol) {
+    u0 = rng::sobol_uniform(sl, k);
+    u1 = rng::sobol_uniform(sl, k + 1);
   } else {
-    rng::uniform_pair(P.seed, P.stream, uint64_t(s - P.start0),
-                      rng::pso_draw(D, a.iter, d, 0) >> 1, u0, u1);
+    rng::uniform_pair(P.seed, P.stream, sl, k >> 1, u0, u1);
   }
 }
 
@@ -275,23 +274,7 @@ pso_coord_apply(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d,
 ZEUS_D ZEUS_FI double
 pso_coord(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d, const double* gb)
 {
-  const int64_t o = s * D + d;
-  if (a.iter == 0) {
-    const double vr = (P.upper - P.lower) * 0.1;
-    const double rx = pso_uniform(a, P, D, s, d, 0, P.lower, P.upper);
-    const double rv = pso_uniform(a, P, D, s, d, 1, -vr, vr);
-    a.X[o] = rx;
-    a.V[o] = rv;
-    a.pbx[o] = rx;
-    return rx;
-  }
-  const double r1 = pso_uniform(a, P, D, s, d, 0, 0.0, 1.0);
-  const double r2 = pso_uniform(a, P, D, s, d, 1, 0.0, 1.0);
-  const double xi = a.X[o], vi = a.V[o], pb = a.pbx[o];
-  const double nv = kPsoW * vi + kPsoC1 * r1 * (pb - xi) + kPsoC2 * r2 * (gb[d] - xi);
-  a.V[o] = nv;
-  a.X[o] = xi + nv;
-  return xi + nv;
+  return pso_coord_apply(a, P, D, s, d, pso_coord_load(a, P, D, s, d), gb);
 }
 
 // Publish the workgroup's best particle into this iteration's partial slot

@@ -192,14 +192,10 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     P.stream = 0;
     P.tol = tol;
     P.x0 = pso ? pbx.data() : nullptr;
-    // XORWOW / SOBOL: per-start states, one PSO iteration's draws, or the
-    // generated BFGS starts (prng.hpp); PHILOX draws in the kernels
+    // XORWOW: one state per (start, coordinate) for the PSO iterations
     P.prng = prng;
-    const bool stateful = prng != rng::kPhilox;
-    HipBuffer<uint32_t> rstate(stateful ? 6 * n : 0);
-    HipBuffer<double> rdraws(stateful && pso ? 2 * n * D : 0), rstarts(stateful && !pso ? n * D : 0);
-    if (stateful && !pso)
-      P.x0 = rstarts.data();
+    const bool xorwow = prng == rng::kXorwow && PSO_ITER > 0;
+    HipBuffer<uint32_t> xw(xorwow ? 6 * n * D : 0);
     const int64_t t_steps = tr ? tr->steps : 0, t_starts = tr ? tr->starts : 0;
     const int64_t t_n = t_steps * t_starts;
     HipBuffer<double> t_coords(t_n * D), t_fval(t_n), t_grad(t_n),
@@ -263,22 +259,14 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
     init_controls_kernel<<<1, 256, 0, s>>>(ctl.data(), 1, gbx.data(), D);
-    if (stateful) {
-      rng_init_kernel<<<nblk, 256, 0, s>>>(dP.data(), 1, n, rstate.data());
-      if (!pso)
-        rng_starts_kernel<<<nblk, 256, 0, s>>>(dP.data(), 1, n, rstate.data(), rstarts.data());
-    }
     hip_check(hipEventRecord(ep, s), "event");
     if (pso) {
       PsoArgs pa{dP.data(),       1,           0,           nblk,           ctl.data(),
                  X.data(),        V.data(),    pbx.data(),  pbv.data(),     gbx.data(),
                  part_val.data(), part_idx.data(), part_x.data(), gstate.data(), bprob.data(),
-                 stateful ? rdraws.data() : nullptr, n};
+                 xorwow ? xw.data() : nullptr, n * D};
       for (int t = 0; t <= PSO_ITER; ++t) {
         pa.iter = t;
-        if (stateful)
-          rng_pso_draws_kernel<<<nblk, 256, 0, s>>>(dP.data(), 1, n, t, rstate.data(),
-                                                    rdraws.data());
         pso_lane_kernel<F, D, UserProvider<F>><<<nblk, 256, 0, s>>>(pa, prov);
       }
     }

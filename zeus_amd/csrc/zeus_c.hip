@@ -353,11 +353,8 @@ struct ClassPlan {
   int ctl0 = 0;          // first control block of the class
   bool traced = false;   // some problem records a trajectory
   bool metrics = false;  // per-start timing requested (ZEUS_FLAG_METRICS)
-  bool rng = false;      // some problem draws XORWOW or SOBOL (prng.hpp)
-  bool gen_starts = false; // ... and some of those draws its BFGS starts
-  uint32_t* rng_state = nullptr; // XORWOW states, 6 words per start
-  double* draws = nullptr;       // one PSO iteration's uniforms, 2D x total
-  double* x0gen = nullptr;       // generated BFGS starts, total x D
+  bool xorwow = false;   // some problem runs PSO on XORWOW streams
+  uint32_t* xw = nullptr; // their (start, coordinate) states: 6 x total x D words
   // host
   std::vector<ProblemDesc> desc;
   std::vector<int32_t> block_problem;
@@ -397,11 +394,8 @@ carve_class(Carver& cv, ClassPlan& c)
   c.aux = cv.take<double>(c.ops->objective == obj::kGaussian ? np * dev::gaussian_aux_doubles(D)
                                                               : 1);
   c.h_slab = cv.take<double>(size_t(std::max<int64_t>(c.slab, 1)));
-  if (c.rng) {
-    c.rng_state = cv.take<uint32_t>(6 * n);
-    c.draws = cv.take<double>(pso ? 2 * n * D : 1);
-    c.x0gen = cv.take<double>(c.gen_starts ? n * D : 1);
-  }
+  if (c.xorwow)
+    c.xw = cv.take<uint32_t>(6 * n * D);
   if (c.metrics) {
     c.out.ad_ticks = cv.take<uint64_t>(n);
     c.out.bfgs_ticks = cv.take<uint64_t>(n);
@@ -424,8 +418,8 @@ pso_args(const ClassPlan& c, Control* ctl)
                  c.pbv,      c.gbx,
                  c.part_val, c.part_idx,
                  c.part_x,   c.gstate,
-                 c.d_block_problem, c.rng ? c.draws : nullptr,
-                 c.total};
+                 c.d_block_problem, c.xw,
+                 c.total * c.ops->dim};
 }
 
 hipError_t
@@ -445,24 +439,10 @@ launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
     if ((e = hipGetLastError()) != hipSuccess)
       return e;
   }
-  const int rg = int((c.total + 255) / 256);
-  if (c.rng && c.total > 0) {
-    dev::rng_init_kernel<<<rg, 256, 0, st>>>(c.d_desc, np, c.total, c.rng_state);
-    if (c.gen_starts)
-      dev::rng_starts_kernel<<<rg, 256, 0, st>>>(c.d_desc, np, c.total, c.rng_state, c.x0gen);
-    if ((e = hipGetLastError()) != hipSuccess)
-      return e;
-  }
   if (c.max_pso_iter >= 0 && c.pso_blocks > 0) {
     PsoArgs pa = pso_args(c, ctl);
     for (int it = 0; it <= c.max_pso_iter; ++it) {
       pa.iter = it;
-      if (c.rng) {
-        dev::rng_pso_draws_kernel<<<rg, 256, 0, st>>>(c.d_desc, np, c.total, it, c.rng_state,
-                                                      c.draws);
-        if ((e = hipGetLastError()) != hipSuccess)
-          return e;
-      }
       if ((e = c.ops->pso(pa, c.pso_blocks, st)) != hipSuccess)
         return e;
     }
@@ -632,10 +612,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       d.n_params = p.n_params;
       d.x0 = p.x0;
       d.prng = p.prng;
-      if (d.prng != ZEUS_PRNG_PHILOX) {
-        c.rng = true;
-        c.gen_starts = c.gen_starts || (d.pso_iter < 0 && d.x0 == nullptr);
-      }
+      c.xorwow = c.xorwow || (d.prng == ZEUS_PRNG_XORWOW && d.pso_iter > 0);
       d.traj = traj_of(p);
       c.traced = c.traced || d.traj.fval != nullptr;
       c.desc.push_back(d);
@@ -715,14 +692,11 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   auto mirror = [&](const void* dev_ptr) {
     return pin + (static_cast<const char*>(dev_ptr) - static_cast<char*>(g_ds.arena));
   };
-  // problems that run PSO start their BFGS from pBestX; stateful generators
-  // without PSO start from the generated points
+  // problems that run PSO start their BFGS from pBestX
   for (auto& c : classes)
     for (auto& d : c.desc) {
       if (d.x0 == nullptr && d.pso_iter >= 0)
         d.x0 = c.pbx + d.start0 * d.dim;
-      else if (d.x0 == nullptr && d.prng != ZEUS_PRNG_PHILOX)
-        d.x0 = c.x0gen + d.start0 * d.dim;
       if (c.ops->objective == obj::kGaussian) {
         d.aux = c.aux + (&d - c.desc.data()) * dev::gaussian_aux_doubles(d.dim);
         d.aux_frag = d.aux + int64_t(d.dim) * d.dim;
@@ -820,7 +794,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       key.push_back(uint64_t(int64_t(c.max_pso_iter)));
       key.push_back(uint64_t(c.probs.size()));
       key.push_back(uint64_t(c.cfg.form) << 32 | uint32_t(c.cfg.threads));
-      key.push_back(uint64_t(c.rng) << 1 | uint64_t(c.gen_starts));
+      key.push_back(uint64_t(c.xorwow));
     }
     hipGraphExec_t exec = nullptr;
     for (const auto& ge : g_ds.graphs)
@@ -974,12 +948,12 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
 // ===========================================================================
 namespace {
 
-// Draws 0..n-1 of one start's stream, in order, by one lane.
+// Draws 0..n-1 of one stream, in order, by one lane.
 __global__ void
 stream_fill_kernel(double* out, int64_t n, int prng, uint32_t seed, uint32_t stream,
-                   uint64_t start)
+                   uint64_t start, uint32_t coord)
 {
-  rng::Xorwow st = rng::xorwow_seed(seed, stream, start);
+  rng::Xorwow st = rng::xorwow_seed(seed, stream, start, coord);
   for (int64_t k = 0; k < n; ++k)
     out[k] = prng == rng::kXorwow  ? rng::xorwow_uniform(st)
              : prng == rng::kSobol ? rng::sobol_uniform(start, uint64_t(k))
@@ -1049,7 +1023,7 @@ zeus_pso_run(const zeus_problem* prob, double* pbest_x, double* pbest_val, doubl
   d.params = prob->params;
   d.n_params = prob->n_params;
   c.desc.push_back(d);
-  c.rng = d.prng != ZEUS_PRNG_PHILOX;
+  c.xorwow = d.prng == ZEUS_PRNG_XORWOW && d.pso_iter > 0;
   c.pso_blocks = int((prob->n_starts + c.ops->pso_ppb - 1) / c.ops->pso_ppb);
   c.block_problem.assign(c.pso_blocks, 0);
   c.total = prob->n_starts;
@@ -1176,7 +1150,7 @@ zeus_philox_fill(double* out, int64_t n, uint32_t seed, uint32_t stream, uint64_
 
 zeus_status_t
 zeus_stream_fill(double* out, int64_t n, int32_t prng, uint32_t seed, uint32_t stream,
-                 uint64_t start, void* stream_handle)
+                 uint64_t start, uint32_t coord, void* stream_handle)
 {
   if (n < 0 || (n > 0 && !out) ||
       (prng != ZEUS_PRNG_XORWOW && prng != ZEUS_PRNG_PHILOX && prng != ZEUS_PRNG_SOBOL))
@@ -1188,7 +1162,7 @@ zeus_stream_fill(double* out, int64_t n, int32_t prng, uint32_t seed, uint32_t s
   if (rc != ZEUS_OK)
     return rc;
   hipStream_t st = stream_handle ? static_cast<hipStream_t>(stream_handle) : g_ds.stream;
-  stream_fill_kernel<<<1, 1, 0, st>>>(out, n, prng, seed, stream, start);
+  stream_fill_kernel<<<1, 1, 0, st>>>(out, n, prng, seed, stream, start, coord);
   hipError_t e = hipGetLastError();
   const hipError_t e2 = hipStreamSynchronize(st);
   return to_status(e != hipSuccess ? e : e2);
