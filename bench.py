@@ -9,13 +9,16 @@ C is uploaded once before the timed region (inputs resident in HBM).
 
 Also measured on the same GPU: the dual-number gradient on the same call
 (the t_dual / t_adjoint ratio), the BFGS kernel against the FP64 peak, and
-the CPU oracle on a bounded sample of the same workload.  --throughput adds
-the no-early-stop rate (required = N + 1); it launches the same BFGS kernel
-on another workload, so it is off by default, where every launch of that
-kernel in the run is a headline launch (its profiled average duration is
-then the one the roofline uses).
+the CPU oracle on a bounded sample of the same workload.  The starts are
+drawn with the default generator (XORWOW, as in the reference); --prng picks
+another.  --throughput adds the no-early-stop rate (required = N + 1) and
+--generators the rate under each generator; both launch the same BFGS
+kernel on other workloads, so they are off by default, where every launch
+of that kernel in the run is a headline launch (its profiled average
+duration is then the one the roofline uses).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--throughput]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--prng G] [--throughput]
+                    [--generators]
 """
 from __future__ import annotations
 
@@ -81,8 +84,7 @@ def cpu_baseline(threads: int):
     from oracle import binding as orc
     C = gauss_matrix().reshape(-1)
     t0 = time.perf_counter()
-    pbx, _, _, _ = orc.pso(orc.GAUSSIAN, D, -5.0, 5.0, N, PSO_ITER, SEED, params=C,
-                           prng=orc.PRNG_PHILOX)
+    pbx, _, _, _ = orc.pso(orc.GAUSSIAN, D, -5.0, 5.0, N, PSO_ITER, SEED, params=C)
     t_pso = time.perf_counter() - t0
     K = 2 * threads
     t0 = time.perf_counter()
@@ -107,6 +109,11 @@ def main() -> int:
     ap.add_argument("--dual-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--throughput", action="store_true")
+    ap.add_argument("--generators", action="store_true",
+                    help="also time the workload under each generator")
+    ap.add_argument("--prng", choices=("xorwow", "philox", "sobol"), default="xorwow",
+                    help="generator of the timed calls (default: the library's and the "
+                         "reference's default, XORWOW)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (multi-GPU sharding is a design "
@@ -122,16 +129,19 @@ def main() -> int:
     f = z.Gaussian(gauss_matrix())
     f.device_params()  # upload C once
 
-    def call(grad, required):
-        return z.Zeus(f, -5.0, 5.0, N, MAX_ITER, PSO_ITER, required, "gaussian", TOL, SEED, 0,
-                      prng=z.PRNGType.PHILOX, gradient=grad)
+    gen = {"xorwow": z.PRNGType.XORWOW, "philox": z.PRNGType.PHILOX,
+           "sobol": z.PRNGType.SOBOL}
 
-    def timed(grad, required, steps, warmup):
+    def call(grad, required, prng=a.prng):
+        return z.Zeus(f, -5.0, 5.0, N, MAX_ITER, PSO_ITER, required, "gaussian", TOL, SEED, 0,
+                      prng=gen[prng], gradient=grad)
+
+    def timed(grad, required, steps, warmup, prng=a.prng):
         for _ in range(warmup):
-            call(grad, required)
+            call(grad, required, prng)
         capi.check(capi.lib().zeus_synchronize(), "sync")
         t0 = time.perf_counter()
-        res = [call(grad, required) for _ in range(steps)]
+        res = [call(grad, required, prng) for _ in range(steps)]
         capi.check(capi.lib().zeus_synchronize(), "sync")
         return res, time.perf_counter() - t0
 
@@ -147,6 +157,15 @@ def main() -> int:
     ms_adj = 1e3 * t_adj / len(adj)
     ms_dual = 1e3 * t_dual / len(dual)
     value_dual = sum(r.c.claimed for r in dual) / t_dual
+    # the same workload under each generator: the call time hardly changes,
+    # the converged count at the early stop does (it depends on how tightly
+    # the PSO result's iteration counts cluster)
+    by_gen = {}
+    for g in (gen if a.generators else ()):
+        rs, tg = timed("adjoint", REQUIRED, max(3, a.steps // 4), 1, g)
+        by_gen[g] = {"value": sum(r.c.claimed for r in rs) / tg,
+                     "ms_per_step": 1e3 * tg / len(rs),
+                     "converged_per_call": float(np.mean([r.c.claimed for r in rs]))}
 
     # roofline of the BFGS kernel (device time from events on its stream)
     form, threads = z.tuned_config(capi.OBJ_GAUSSIAN, D, capi.GRAD_ADJOINT)
@@ -175,7 +194,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "gaussian-150 multistart: 4096 starts, PSO 10, MAX_ITER 10000, "
                                "tol 1e-8, required 100, seed 42, adjoint gradient, "
-                               "Philox generator (PRNGType.PHILOX)",
+                               f"{a.prng} generator",
+                   "prng": a.prng,
                    "dim": D, "starts": N, "required": REQUIRED, "pso_iter": PSO_ITER,
                    "parallelism": layout},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
@@ -188,6 +208,7 @@ def main() -> int:
                  "converged_per_call": dual[0].c.claimed},
         "phases_ms": {"pso": float(np.mean([r.ms_pso for r in adj])), "bfgs": ms_opt},
         "converged_per_call": adj[0].c.claimed,
+        **({"by_generator": by_gen} if by_gen else {}),
         "best_fval": adj[0].fval,
     }
     if a.throughput:
