@@ -213,6 +213,39 @@ zeus_status_t zeus_pso_run(const zeus_problem* prob, double* pbest_x,
                            double* pbest_val, double* gbest_x,
                            double* gbest_val, void* stream);
 
+/* The swarm of one problem, caller-owned device buffers: positions and
+   velocities (N x D, start-major), pBest coordinates and values (N x D, N),
+   gBest coordinates and value (D, 1), and for the XORWOW generator the
+   (start, coordinate) states (6 x N x D uint32; NULL otherwise). */
+typedef struct zeus_swarm {
+  double* x;
+  double* v;
+  double* pbest_x;
+  double* pbest_val;
+  double* gbest_x;
+  double* gbest_val;
+  uint32_t* rng;
+} zeus_swarm;
+
+/* PSO phases (reference pso::initKernel and pso::iterKernel,
+   include/pso.cuh:17-155): zeus_pso_init draws positions and velocities,
+   sets pBest = X and computes gBest; zeus_pso_step runs iteration iter
+   (>= 1) with the swarm's gBest frozen, then updates pBest (strict
+   improvement) and gBest.  init followed by steps 1 .. T gives the same
+   swarm as zeus_pso_run with pso_iter = T. */
+zeus_status_t zeus_pso_init(const zeus_problem* prob, const zeus_swarm* swarm,
+                            void* stream);
+zeus_status_t zeus_pso_step(const zeus_problem* prob, int32_t iter,
+                            const zeus_swarm* swarm, void* stream);
+
+/* BFGS update of a batch of n x n matrices (device, row-major) with device
+   vectors s, y (batch x n each): reference util::bfgsUpdate
+   (include/utils.cuh:245-287), skipped when |s.y| < 1e-14, in the O(n^2)
+   rank-2 form H' = H + s a^T + a s^T (a = (beta/2) s - rho H y, beta =
+   rho^2 y.Hy + rho, rho = 1/s.y).  n <= 1024. */
+zeus_status_t zeus_bfgs_update(double* H, const double* s, const double* y,
+                               int64_t batch, int32_t n, void* stream);
+
 /* Objective values (and gradients when grad != NULL) at n points (device
    arrays, x: n x dim), lane per point; gradient ZEUS_GRAD_*. */
 zeus_status_t zeus_eval(const zeus_problem* prob, const double* x, int64_t n,

@@ -10,7 +10,7 @@ from .api import (Ackley, Convergence, DeviceArray, Gaussian, LogLikelihood, Mlp
                   PRNGType, Problem, Rastrigin, Result, Rosenbrock, Sphere, Zeus, argmin,
                   evaluate, make_dijet, make_mlp_data, make_spd, philox_fill, philox_uniform,
                   pso, run_batch, set_fault_hooks, tuned_config, write_trajectory,
-                  euclidean_error, append_results_tsv, stream_uniforms)
+                  euclidean_error, append_results_tsv, stream_uniforms, Swarm, bfgs_update)
 from . import _capi as capi
 
 __all__ = [
@@ -18,5 +18,6 @@ __all__ = [
     "PRNGType", "Problem", "Rastrigin", "Result", "Rosenbrock", "Sphere", "Zeus", "argmin",
     "evaluate", "make_dijet", "make_mlp_data", "make_spd", "philox_fill", "philox_uniform",
     "pso", "run_batch", "set_fault_hooks", "tuned_config", "write_trajectory",
-    "euclidean_error", "append_results_tsv", "stream_uniforms", "capi",
+    "euclidean_error", "append_results_tsv", "stream_uniforms", "Swarm", "bfgs_update",
+    "capi",
 ]

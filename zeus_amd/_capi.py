@@ -30,6 +30,12 @@ class Trajectory(C.Structure):
                 ("starts", C.c_int32)]
 
 
+class Swarm(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("v", C.c_void_p), ("pbest_x", C.c_void_p),
+                ("pbest_val", C.c_void_p), ("gbest_x", C.c_void_p), ("gbest_val", C.c_void_p),
+                ("rng", C.c_void_p)]
+
+
 class Problem(C.Structure):
     _fields_ = [("objective", C.c_int32), ("dim", C.c_int32), ("lower", C.c_double),
                 ("upper", C.c_double), ("n_starts", C.c_int64), ("max_iter", C.c_int32),
@@ -78,6 +84,10 @@ SIGNATURES = {
                                             C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                             C.POINTER(Result), _dp]),
+    "zeus_pso_init": (C.c_int32, [C.POINTER(Problem), C.POINTER(Swarm), C.c_void_p]),
+    "zeus_pso_step": (C.c_int32, [C.POINTER(Problem), C.c_int32, C.POINTER(Swarm), C.c_void_p]),
+    "zeus_bfgs_update": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                     C.c_void_p]),
     "zeus_stream_fill": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, C.c_uint32, C.c_uint32,
                                      C.c_uint64, C.c_uint32, C.c_void_p]),
     "zeus_write_trajectory": (C.c_int32, [C.c_char_p, _dp, _dp, _dp, C.c_void_p, _dp,

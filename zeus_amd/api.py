@@ -26,6 +26,7 @@ __all__ = [
     "Problem", "run_batch", "pso", "evaluate", "argmin", "philox_uniform", "philox_fill",
     "make_dijet", "make_mlp_data", "make_spd", "tuned_config", "set_fault_hooks",
     "write_trajectory", "euclidean_error", "append_results_tsv", "stream_uniforms",
+    "Swarm", "bfgs_update",
 ]
 
 
@@ -462,6 +463,61 @@ def pso(f: Objective, lower, upper, N, PSO_ITER, seed, stream=0, prng=PRNGType.X
     gv = DeviceArray(nbytes=8)
     check(lib().zeus_pso_run(C.byref(P), px.ptr, pv.ptr, gx.ptr, gv.ptr, None), "zeus_pso_run")
     return (px.to_host().reshape(N, f.dim), pv.to_host(), gx.to_host(), float(gv.to_host()[0]))
+
+
+class Swarm:
+    """Device swarm of one problem for the PSO phases (zeus_pso_init /
+    zeus_pso_step; reference pso::initKernel / iterKernel)."""
+
+    def __init__(self, f: Objective, lower, upper, N, seed, stream=0,
+                 prng=PRNGType.XORWOW):
+        self.f, self.N, self.D = f, int(N), f.dim
+        addr, nparam = f.device_params()
+        self.problem = capi.Problem(f.kind, f.dim, float(lower), float(upper), int(N), 0, 0, 0,
+                                    int(seed) & 0xFFFFFFFF, int(stream) & 0xFFFFFFFF, int(prng),
+                                    0.0, addr or None, nparam, None, None, None)
+        n, d = self.N, self.D
+        self.x = DeviceArray(nbytes=8 * n * d)
+        self.v = DeviceArray(nbytes=8 * n * d)
+        self.pbx = DeviceArray(nbytes=8 * n * d)
+        self.pbv = DeviceArray(nbytes=8 * n)
+        self.gbx = DeviceArray(nbytes=8 * d)
+        self.gbv = DeviceArray(nbytes=8)
+        self.rng = DeviceArray(nbytes=4 * 6 * n * d, dtype=np.uint32) \
+            if int(prng) == PRNGType.XORWOW else None
+        self.c = capi.Swarm(self.x.addr or None, self.v.addr or None, self.pbx.addr or None,
+                            self.pbv.addr or None, self.gbx.addr or None,
+                            self.gbv.addr or None,
+                            (self.rng.addr or None) if self.rng is not None else None)
+
+    def init(self):
+        check(lib().zeus_pso_init(C.byref(self.problem), C.byref(self.c), None),
+              "zeus_pso_init")
+        return self
+
+    def step(self, it: int):
+        check(lib().zeus_pso_step(C.byref(self.problem), int(it), C.byref(self.c), None),
+              "zeus_pso_step")
+        return self
+
+    def state(self):
+        """pBestX, pBestVal, gBestX, gBestVal (and X, V) on the host."""
+        n, d = self.N, self.D
+        return dict(pbest_x=self.pbx.to_host().reshape(n, d), pbest_val=self.pbv.to_host(),
+                    gbest_x=self.gbx.to_host(), gbest_val=float(self.gbv.to_host()[0]),
+                    x=self.x.to_host().reshape(n, d), v=self.v.to_host().reshape(n, d))
+
+
+def bfgs_update(H, s, y) -> np.ndarray:
+    """Batched BFGS update on the device (zeus_bfgs_update): H (B x n x n),
+    s and y (B x n); returns the updated matrices."""
+    H = np.ascontiguousarray(H, np.float64)
+    s = np.ascontiguousarray(s, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    B, n = s.shape
+    dH, ds, dy = DeviceArray(H), DeviceArray(s), DeviceArray(y)
+    check(lib().zeus_bfgs_update(dH.ptr, ds.ptr, dy.ptr, B, n, None), "zeus_bfgs_update")
+    return dH.to_host().reshape(B, n, n)
 
 
 def evaluate(f: Objective, X, gradient: str | None = None):

@@ -50,6 +50,7 @@ struct PsoArgs {
   // k at xw[k * xw_stride + s * D + d]; nullptr when no problem uses XORWOW
   uint32_t* xw;
   int64_t xw_stride;
+  int32_t keep_rng; // store the states after the last iteration too (PSO phases)
 };
 
 // (value, index) order for argmin: NaN loses to any number, ties and the
@@ -217,7 +218,7 @@ pso_uniforms(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d, do
                                  : xw_load(a.xw, a.xw_stride, o);
     u0 = rng::xorwow_uniform(st);
     u1 = rng::xorwow_uniform(st);
-    if (a.iter < P.pso_iter)
+    if (a.iter < P.pso_iter || a.keep_rng)
       xw_store(a.xw, a.xw_stride, o, st);
   } else if (P.prng == rng::kSobol) {
     u0 = rng::sobol_uniform(sl, k);
@@ -539,6 +540,31 @@ pso_gaussian_mfma_kernel(PsoArgs a)
       if (j == best_j)
         px[d] = (improved[j] || a.iter == 0) ? xs[d * S::XS + j] : a.pbx[s * D + d];
     }
+  }
+}
+
+// Make the fold of iteration t_prev + 1 of problem 0 return the caller's
+// gBest (value *gbv, coordinates a.gbx): it is the only partial of
+// iteration t_prev (slot 0; the other slots hold NaN, which loses every
+// comparison) and also the gBest stored as the one used during iteration
+// t_prev, so the fold keeps it whichever of the two it compares (before
+// iteration 1 the stored one is not read).  One workgroup.
+static __global__ void
+pso_seed_kernel(PsoArgs a, int t_prev, const double* gbv, int D)
+{
+  const int q = t_prev & 1;
+  const double v = *gbv;
+  for (int b = threadIdx.x; b < a.n_blocks; b += blockDim.x) {
+    a.part_val[int64_t(q) * a.n_blocks + b] = b == 0 ? v : __builtin_nan("");
+    a.part_idx[int64_t(q) * a.n_blocks + b] = b == 0 ? 0 : int64_t(1) << 62;
+  }
+  double* g = a.gstate + (q ^ 1) * (D + 1);
+  double* px = a.part_x + int64_t(q) * a.n_blocks * D;
+  if (threadIdx.x == 0)
+    g[0] = v;
+  for (int d = threadIdx.x; d < D; d += blockDim.x) {
+    g[1 + d] = a.gbx[d];
+    px[d] = a.gbx[d];
   }
 }
 

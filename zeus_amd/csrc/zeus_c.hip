@@ -354,6 +354,7 @@ struct ClassPlan {
   bool traced = false;   // some problem records a trajectory
   bool metrics = false;  // per-start timing requested (ZEUS_FLAG_METRICS)
   bool xorwow = false;   // some problem runs PSO on XORWOW streams
+  int keep_rng = 0;      // PSO phases: the caller keeps the XORWOW states
   uint32_t* xw = nullptr; // their (start, coordinate) states: 6 x total x D words
   // host
   std::vector<ProblemDesc> desc;
@@ -419,11 +420,12 @@ pso_args(const ClassPlan& c, Control* ctl)
                  c.part_val, c.part_idx,
                  c.part_x,   c.gstate,
                  c.d_block_problem, c.xw,
-                 c.total * c.ops->dim};
+                 c.total * c.ops->dim, c.keep_rng};
 }
 
+// Objective preparation and PSO iterations t0 .. max_pso_iter of a class.
 hipError_t
-launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
+launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st, int t0 = 0)
 {
   const int D = c.ops->dim;
   const int np = int(c.probs.size());
@@ -441,7 +443,7 @@ launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
   }
   if (c.max_pso_iter >= 0 && c.pso_blocks > 0) {
     PsoArgs pa = pso_args(c, ctl);
-    for (int it = 0; it <= c.max_pso_iter; ++it) {
+    for (int it = t0; it <= c.max_pso_iter; ++it) {
       pa.iter = it;
       if ((e = c.ops->pso(pa, c.pso_blocks, st)) != hipSuccess)
         return e;
@@ -986,6 +988,137 @@ philox_fill_kernel(double* out, int64_t n, uint32_t seed, uint32_t stream, uint6
     out[i] = rng::uniform(seed, stream, start, draw0 + uint64_t(i));
 }
 
+// Rank-2 BFGS update of a batch of n x n matrices H[b] (row-major) with
+// s[b], y[b] (reference utils.cuh:245-287 in its O(n^2) form: skipped when
+// |s.y| < 1e-14; H' = H + s a^T + a s^T, a = (beta/2) s - rho H y, beta =
+// rho^2 y.Hy + rho), one workgroup per matrix, sums in the oracle's order.
+__global__ void
+bfgs_update_kernel(double* H, const double* s, const double* y, int n)
+{
+  extern __shared__ double sm[];
+  __shared__ double red[2];
+  const int64_t b = blockIdx.x;
+  double* h = H + b * int64_t(n) * n;
+  const double* sb = s + b * n;
+  const double* yb = y + b * n;
+  double* u = sm;     // H y
+  double* a = sm + n; // the update vector
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j)
+      acc = ::fma(h[int64_t(i) * n + j], yb[j], acc);
+    u[i] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) { // left-to-right sums, as the oracle's dot products
+    double sty = 0.0, yu = 0.0;
+    for (int j = 0; j < n; ++j)
+      sty = ::fma(sb[j], yb[j], sty);
+    for (int j = 0; j < n; ++j)
+      yu = ::fma(yb[j], u[j], yu);
+    red[0] = sty;
+    red[1] = yu;
+  }
+  __syncthreads();
+  const double sty = red[0];
+  if (::fabs(sty) < 1e-14)
+    return; // uniform over the workgroup
+  const double rho = 1.0 / sty;
+  const double hb = 0.5 * ::fma(rho * rho, red[1], rho);
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    a[i] = ::fma(hb, sb[i], -(rho * u[i]));
+  __syncthreads();
+  for (int64_t e = threadIdx.x; e < int64_t(n) * n; e += blockDim.x) {
+    const int i = int(e / n), j = int(e % n);
+    h[e] = h[e] + (sb[i] * a[j] + a[i] * sb[j]);
+  }
+}
+
+// PSO iterations t0 .. t1 of one problem on the given buffers (any of X, V,
+// xw may be null: then the arena holds them).  With `seed`, iteration t0
+// (>= 1) starts from the caller's gBest (gbx, gbv) instead of the partials
+// of iteration t0 - 1.  Ends with the final fold: gbx and *gbv hold the
+// gBest after iteration t1.
+zeus_status_t
+pso_phase(const zeus_problem* prob, int t0, int t1, double* X, double* V, double* pbx,
+          double* pbv, double* gbx, double* gbv, uint32_t* xw, bool seed, hipStream_t st)
+{
+  ClassPlan c;
+  c.ops = rt::find_class(prob->objective, prob->dim);
+  c.probs.push_back(0);
+  ProblemDesc d{};
+  d.objective = prob->objective;
+  d.dim = prob->dim;
+  d.lower = prob->lower;
+  d.upper = prob->upper;
+  d.n_starts = prob->n_starts;
+  d.pso_iter = t1;
+  d.seed = prob->seed;
+  d.stream = prob->stream;
+  d.prng = prob->prng;
+  d.params = prob->params;
+  d.n_params = prob->n_params;
+  c.desc.push_back(d);
+  c.xorwow = d.prng == ZEUS_PRNG_XORWOW && t1 > 0 && xw == nullptr;
+  c.pso_blocks = int((prob->n_starts + c.ops->pso_ppb - 1) / c.ops->pso_ppb);
+  c.block_problem.assign(c.pso_blocks, 0);
+  c.total = prob->n_starts;
+  c.max_pso_iter = t1;
+  Control* d_ctl;
+  auto layout = [&](Carver& cv) {
+    carve_upload(cv, c);
+    carve_class(cv, c);
+    d_ctl = cv.take<Control>(1);
+  };
+  zeus_status_t rc;
+  {
+    Carver m{nullptr};
+    layout(m);
+    if ((rc = arena_reserve(m.off)) != ZEUS_OK)
+      return rc;
+    Carver cv{static_cast<char*>(g_ds.arena)};
+    layout(cv);
+  }
+  if (X)
+    c.X = X;
+  if (V)
+    c.V = V;
+  if (xw) {
+    c.xw = xw;
+    c.keep_rng = 1;
+  }
+  c.pbx = pbx;
+  c.pbv = pbv;
+  if (gbx)
+    c.gbx = gbx;
+  if (c.ops->objective == obj::kGaussian) {
+    c.desc[0].aux = c.aux;
+    c.desc[0].aux_frag = c.aux + int64_t(prob->dim) * prob->dim;
+  }
+  hipError_t e = hipMemcpyAsync(c.d_desc, c.desc.data(), sizeof(ProblemDesc),
+                                hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c.d_block_problem, c.block_problem.data(), 4 * c.block_problem.size(),
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    dev::init_controls_kernel<<<1, 256, 0, st>>>(d_ctl, 1, seed ? nullptr : c.gbx,
+                                                 seed ? 0 : prob->dim);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && seed) { // the fold of iteration t0 keeps the caller's gBest
+    dev::pso_seed_kernel<<<1, 256, 0, st>>>(pso_args(c, d_ctl), t0 - 1, gbv, prob->dim);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = launch_class(c, d_ctl, 0, st, t0);
+  if (e == hipSuccess)
+    e = c.ops->pso_final(pso_args(c, d_ctl), 1, st);
+  if (e == hipSuccess && gbv)
+    e = hipMemcpyAsync(gbv, &d_ctl->gbest_val, sizeof(double), hipMemcpyDeviceToDevice, st);
+  const hipError_t e2 = hipStreamSynchronize(st);
+  return to_status(e != hipSuccess ? e : e2);
+}
+
 } // namespace
 
 extern "C" {
@@ -1007,65 +1140,66 @@ zeus_pso_run(const zeus_problem* prob, double* pbest_x, double* pbest_val, doubl
   if ((rc = ensure_device()) != ZEUS_OK)
     return rc;
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g_ds.stream;
-  ClassPlan c;
-  c.ops = rt::find_class(prob->objective, prob->dim);
-  c.probs.push_back(0);
-  ProblemDesc d{};
-  d.objective = prob->objective;
-  d.dim = prob->dim;
-  d.lower = prob->lower;
-  d.upper = prob->upper;
-  d.n_starts = prob->n_starts;
-  d.pso_iter = prob->pso_iter;
-  d.seed = prob->seed;
-  d.stream = prob->stream;
-  d.prng = prob->prng;
-  d.params = prob->params;
-  d.n_params = prob->n_params;
-  c.desc.push_back(d);
-  c.xorwow = d.prng == ZEUS_PRNG_XORWOW && d.pso_iter > 0;
-  c.pso_blocks = int((prob->n_starts + c.ops->pso_ppb - 1) / c.ops->pso_ppb);
-  c.block_problem.assign(c.pso_blocks, 0);
-  c.total = prob->n_starts;
-  c.max_pso_iter = prob->pso_iter;
-  Control* d_ctl;
-  auto layout = [&](Carver& cv) {
-    carve_upload(cv, c);
-    carve_class(cv, c);
-    d_ctl = cv.take<Control>(1);
-  };
-  {
-    Carver m{nullptr};
-    layout(m);
-    if ((rc = arena_reserve(m.off)) != ZEUS_OK)
-      return rc;
-    Carver cv{static_cast<char*>(g_ds.arena)};
-    layout(cv);
-  }
-  c.pbx = pbest_x;
-  c.pbv = pbest_val;
-  if (gbest_x)
-    c.gbx = gbest_x;
-  if (c.ops->objective == obj::kGaussian) {
-    c.desc[0].aux = c.aux;
-    c.desc[0].aux_frag = c.aux + int64_t(prob->dim) * prob->dim;
-  }
-  hipError_t e = hipMemcpyAsync(c.d_desc, c.desc.data(), sizeof(ProblemDesc),
-                                hipMemcpyHostToDevice, st);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(c.d_block_problem, c.block_problem.data(), 4 * c.block_problem.size(),
-                       hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) {
-    dev::init_controls_kernel<<<1, 256, 0, st>>>(d_ctl, 1, c.gbx, prob->dim);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess)
-    e = launch_class(c, d_ctl, 0, st);
-  if (e == hipSuccess)
-    e = c.ops->pso_final(pso_args(c, d_ctl), 1, st);
-  if (e == hipSuccess && gbest_val)
-    e = hipMemcpyAsync(gbest_val, &d_ctl->gbest_val, sizeof(double), hipMemcpyDeviceToDevice,
-                       st);
+  return pso_phase(prob, 0, prob->pso_iter, nullptr, nullptr, pbest_x, pbest_val, gbest_x,
+                   gbest_val, nullptr, false, st);
+}
+
+zeus_status_t
+zeus_pso_init(const zeus_problem* prob, const zeus_swarm* sw, void* stream)
+{
+  if (!prob || !sw || !sw->x || !sw->v || !sw->pbest_x || !sw->pbest_val || !sw->gbest_x ||
+      !sw->gbest_val || (prob->prng == ZEUS_PRNG_XORWOW && !sw->rng))
+    return ZEUS_E_ARG;
+  zeus_status_t rc = validate(*prob);
+  if (rc != ZEUS_OK)
+    return rc;
+  if (prob->n_starts == 0)
+    return ZEUS_OK;
+  std::lock_guard<std::mutex> lock(g_mu);
+  if ((rc = ensure_device()) != ZEUS_OK)
+    return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g_ds.stream;
+  // iteration 0 seeds the XORWOW states; they are kept for the steps
+  return pso_phase(prob, 0, 0, sw->x, sw->v, sw->pbest_x, sw->pbest_val, sw->gbest_x,
+                   sw->gbest_val, sw->rng, false, st);
+}
+
+zeus_status_t
+zeus_pso_step(const zeus_problem* prob, int32_t iter, const zeus_swarm* sw, void* stream)
+{
+  if (!prob || !sw || iter < 1 || !sw->x || !sw->v || !sw->pbest_x || !sw->pbest_val ||
+      !sw->gbest_x || !sw->gbest_val || (prob->prng == ZEUS_PRNG_XORWOW && !sw->rng))
+    return ZEUS_E_ARG;
+  zeus_status_t rc = validate(*prob);
+  if (rc != ZEUS_OK)
+    return rc;
+  if (prob->n_starts == 0)
+    return ZEUS_OK;
+  std::lock_guard<std::mutex> lock(g_mu);
+  if ((rc = ensure_device()) != ZEUS_OK)
+    return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g_ds.stream;
+  return pso_phase(prob, iter, iter, sw->x, sw->v, sw->pbest_x, sw->pbest_val, sw->gbest_x,
+                   sw->gbest_val, sw->rng, true, st);
+}
+
+zeus_status_t
+zeus_bfgs_update(double* H, const double* s, const double* y, int64_t batch, int32_t n,
+                 void* stream)
+{
+  if (batch < 0 || n <= 0 || n > 1024 || (batch > 0 && (!H || !s || !y)))
+    return ZEUS_E_ARG;
+  if (batch == 0)
+    return ZEUS_OK;
+  std::lock_guard<std::mutex> lock(g_mu);
+  zeus_status_t rc = ensure_device();
+  if (rc != ZEUS_OK)
+    return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g_ds.stream;
+  if (batch > (int64_t(1) << 31) - 1)
+    return ZEUS_E_ARG;
+  bfgs_update_kernel<<<int(batch), 256, size_t(2) * n * sizeof(double), st>>>(H, s, y, n);
+  hipError_t e = hipGetLastError();
   const hipError_t e2 = hipStreamSynchronize(st);
   return to_status(e != hipSuccess ? e : e2);
 }
