@@ -72,3 +72,16 @@ def test_bfgs_update_kat_and_oracle(orc):
             sty = orc.dot(s[b], y[b])
             assert np.array_equal(got[b], orc.update(H0[b], s[b], y[b], sty)), (n, b)
         assert np.array_equal(got[1], H0[1])
+
+
+def test_pso_init_single_particle():
+    # reference test/pso_tests.cu:27-94: with N = 1 the initial pBest is the
+    # particle, pBestVal = gBestVal = f(pBestX) and gBestX = pBestX
+    sw = z.Swarm(z.Rastrigin(2), -5.0, 5.0, 1, 42).init()
+    st = sw.state()
+    x = st["pbest_x"][0]
+    want = 20.0 + sum(v * v - 10.0 * np.cos(2 * np.pi * v) for v in x)
+    assert np.all((x >= -5.0) & (x <= 5.0)) and np.array_equal(st["x"][0], x)
+    assert st["pbest_val"][0] == pytest.approx(want, abs=1e-12)
+    assert st["gbest_val"] == st["pbest_val"][0]
+    assert np.array_equal(st["gbest_x"], x)
