@@ -404,9 +404,9 @@ pso_group_kernel(PsoArgs a, Prov prov)
 }
 
 // Gaussian form: a 1024-thread workgroup owns 16 particles.  Their objective
-// values come from G = Cs Xt on FP64 MFMA (cs_product.hpp), Xt the 16
-// positions in LDS; f_i = 0.5 x_i . (Cs x_i), the per-unit partial dot
-// products added in a fixed order.
+// values f_i = 0.5 x_i . (Cs x_i) come from the upper block triangle of Cs on
+// FP64 MFMA (cs_product.hpp, CsUpperSeg), Xt the 16 positions in LDS; the
+// per-segment partial dot products are added in a fixed order.
 constexpr int kPsoTile = kCsTile; // particles per workgroup (MFMA N)
 constexpr int kPsoTileThreads = 1024;
 
@@ -416,8 +416,9 @@ pso_gaussian_mfma_kernel(PsoArgs a)
 {
   using S = CsShape<D>;
   constexpr int NW = kPsoTileThreads / kWave;
+  using Seg = CsUpperSeg<D, CsUpper<D>::template na<NW>()>;
   __shared__ double xs[S::KP * S::XS]; // positions, [dim][particle]
-  __shared__ double fpart[S::U * kPsoTile];
+  __shared__ double fpart[Seg::NF * kPsoTile];
   __shared__ double gb[D];
   __shared__ double pbv[kPsoTile];
   __shared__ int improved[kPsoTile];
@@ -433,16 +434,15 @@ pso_gaussian_mfma_kernel(PsoArgs a)
 
   for (int e = tid; e < S::KP * S::XS; e += kPsoTileThreads)
     xs[e] = 0.0;
-  // the fold's partials are read first; then the Cs fragments of this
-  // wavefront's first product unit and this thread's coordinate operands and
-  // draws (coalesced over d), none of which depends on gBest, so that their
-  // latency overlaps the rest of the fold (loads complete in issue order)
+  // the fold's partials are read first; then this wavefront's fragments of
+  // the product and this thread's coordinate operands and draws (coalesced
+  // over d), none of which depends on gBest, so that their latency overlaps
+  // the rest of the fold (loads complete in issue order)
   FoldIn fin{};
   if (a.iter > 0)
     fin = pso_fold_read<kPsoTileThreads, kPsoTile, D>(a, pi, P, a.iter - 1);
-  CsFrag<D> frag0;
-  if (w < S::U)
-    frag0.load(P.aux_frag, w, li, lk);
+  Seg seg;
+  seg.load(w, P.aux_frag + CsUpper<D>::kFragOffset, li, lk);
   constexpr int kE = (kPsoTile * D + kPsoTileThreads - 1) / kPsoTileThreads;
   PsoCoordIn cin[kE];
 #pragma unroll
@@ -459,6 +459,11 @@ pso_gaussian_mfma_kernel(PsoArgs a)
     (void)pso_fold_finish<kPsoTileThreads, kPsoTile, D>(a, pi, P, a.iter - 1, gb,
                                                         int(blockIdx.x) == P.block0, fin);
   __syncthreads();
+  // Every load issued so far (fragments, operands) is complete before the
+  // coordinate stores are issued: vector memory counts loads and stores
+  // together, so a fragment still counted at the first MFMA would make it wait
+  // for these stores as well.  The operands are needed here anyway.
+  __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0) (expcnt, lgkmcnt untouched)
   // update / init coordinates and stage the tile
 #pragma unroll
   for (int k = 0; k < kE; ++k) {
@@ -469,24 +474,7 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   }
   __syncthreads();
 
-  static_assert(S::U <= 2 * NW, "at most two product units per wavefront");
-  CsFrag<D> frag1;
-  if (w + NW < S::U)
-    frag1.load(P.aux_frag, w + NW, li, lk); // in flight behind the first chain
-  for (int u = w; u < S::U; u += NW) {
-    const mfma_acc_t acc = u == w ? frag0.mma(xs, u, li, lk) : frag1.mma(xs, u, li, lk);
-    double fp = 0.0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int rr = 16 * (u >> 1) + 4 * q + lk;
-      if (rr < D)
-        fp = ::fma(xs[rr * S::XS + li], acc[q], fp);
-    }
-    fp += __shfl_xor(fp, 16, kWave);
-    fp += __shfl_xor(fp, 32, kWave);
-    if (lk == 0)
-      fpart[u * kPsoTile + li] = fp;
-  }
+  seg.mma(w, xs, fpart, li, lk);
   __syncthreads();
 
   // f_i, pBest on strict improvement
@@ -495,10 +483,10 @@ pso_gaussian_mfma_kernel(PsoArgs a)
     int imp = 0;
     double v = __builtin_nan("");
     if (sl < P.n_starts) {
-      double t = 0.0;
-      for (int u = 0; u < S::U; ++u)
-        t += fpart[u * kPsoTile + tid];
-      const double fv = 0.5 * t;
+      double fv = 0.0; // the upper form carries the factor 0.5
+#pragma unroll
+      for (int k = 0; k < Seg::NF; ++k)
+        fv += fpart[k * kPsoTile + tid];
       const int64_t s = P.start0 + sl;
       if (a.iter == 0) {
         a.pbv[s] = fv;

@@ -74,13 +74,37 @@ symmetrize_kernel(const double* C, double* Cs, double* Cf, int D)
     const int r = 16 * mb + (lane & 15), c = 4 * kb + (lane >> 4);
     Cf[e] = (r < D && c < D) ? 0.5 * (C[int64_t(r) * D + c] + C[int64_t(c) * D + r]) : 0.0;
   }
+  // upper fragments (cs_product.hpp, CsUpper): row block mb keeps k steps
+  // 4 mb .. KB - 1, the diagonal block (k steps 4 mb .. 4 mb + 3) times 0.5
+  int64_t nu = 0;
+  for (int mb = 0; mb < MB; ++mb)
+    nu += KB - 4 * mb;
+  if (e < nu * 64) {
+    const int lane = int(e & 63);
+    int64_t f = e >> 6;
+    int mb = 0;
+    while (f >= KB - 4 * mb) {
+      f -= KB - 4 * mb;
+      ++mb;
+    }
+    const int kb = 4 * mb + int(f);
+    const int r = 16 * mb + (lane & 15), c = 4 * kb + (lane >> 4);
+    const double v =
+      (r < D && c < D) ? 0.5 * (C[int64_t(r) * D + c] + C[int64_t(c) * D + r]) : 0.0;
+    Cf[int64_t(MB) * KB * 64 + e] = kb < 4 * mb + 4 ? 0.5 * v : v;
+  }
 }
 
-// Doubles of the Gaussian preparation per problem: Cs, then its fragments.
+// Doubles of the Gaussian preparation per problem: Cs, its fragments, then
+// the upper fragments.
 ZEUS_HD constexpr int64_t
 gaussian_aux_doubles(int D)
 {
-  return int64_t(D) * D + int64_t((D + 15) / 16) * ((D + 3) / 4) * 64;
+  const int KB = (D + 3) / 4, MB = (D + 15) / 16;
+  int64_t nu = 0;
+  for (int mb = 0; mb < MB; ++mb)
+    nu += KB - 4 * mb;
+  return int64_t(D) * D + (int64_t(MB) * KB + nu) * 64;
 }
 
 // One workgroup per problem: deterministic argmin of the start values (the
