@@ -281,8 +281,9 @@ struct GroupEvalLogLik {
 // in a fixed order: the 65 per-neuron terms (output weights and bias,
 // hidden biases, first-layer weights) by wavefront sums and a per-wavefront
 // fold, the 256 hidden-to-hidden weights as dot products over the points
-// staged in LDS (lane k*16 + j owns weight (k, j)).  The dual gradient keeps
-// the reference's D forward passes spread over the lanes.
+// staged in LDS (lane k*16 + j owns weight (k, j)), half a workgroup's points
+// at a time so that the staging fits two workgroups per CU.  The dual
+// gradient keeps the reference's D forward passes spread over the lanes.
 template <int NT, Gradient G>
 struct GroupEvalMlp {
   using F = obj::Mlp;
@@ -338,8 +339,9 @@ struct GroupEvalMlp {
     if constexpr (G == Gradient::Dual) {
       group_dual_gradient<F, int(F::arity), NT>(f, th, g);
     } else {
-      __shared__ double h1s[NT][H + 1]; // one point per lane, padded rows
-      __shared__ double d2s[NT][H + 1];
+      constexpr int kStage = NT / 2; // points staged at a time
+      __shared__ double h1s[kStage][H + 1]; // one point per lane, padded rows
+      __shared__ double d2s[kStage][H + 1];
       __shared__ double colw[NT / kWave][kCols];
       const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
       const double scale = 2.0 / double(f.m);
@@ -383,19 +385,26 @@ struct GroupEvalMlp {
           col_add(2 * H + 1 + j, dj * xp);                         // w1
           col_add(3 * H + 1 + j, dj);                              // b1
         }
+        // points c0 .. c0 + NT - 1 in two halves, in point order
+#pragma unroll 1
+        for (int half = 0; half < 2; ++half) {
+          if (tid / kStage == half) {
 #pragma unroll
-        for (int j = 0; j < H; ++j) {
-          h1s[tid][j] = h1[j];
-          d2s[tid][j] = d2[j];
+            for (int j = 0; j < H; ++j) {
+              h1s[tid % kStage][j] = h1[j];
+              d2s[tid % kStage][j] = d2[j];
+            }
+          }
+          __syncthreads();
+          if (tid < H * H) {
+            const int k = tid / H, j = tid % H;
+            const int left = f.m - c0 - half * kStage;
+            const int n = left < kStage ? (left > 0 ? left : 0) : kStage;
+            for (int q = 0; q < n; ++q)
+              gw = ::fma(d2s[q][k], h1s[q][j], gw);
+          }
+          __syncthreads();
         }
-        __syncthreads();
-        if (tid < H * H) {
-          const int k = tid / H, j = tid % H;
-          const int n = f.m - c0 < NT ? f.m - c0 : NT;
-          for (int q = 0; q < n; ++q)
-            gw = ::fma(d2s[q][k], h1s[q][j], gw);
-        }
-        __syncthreads();
       }
       for (int c = tid; c < kCols; c += NT) {
         double t = 0.0;
@@ -673,8 +682,21 @@ group_bfgs(const F& f, GroupVectors<D, NT>& L, DistMatrix<D, NT, HREG>& H, Eval&
   __syncthreads();
 }
 
+// Resident workgroups per CU the kernel is compiled for: the MLP with the
+// adjoint gradient alternates evaluation phases (no memory traffic) with the
+// streamed H pass (HBM bound), so two workgroups per CU overlap one's pass
+// with the other's evaluation; the register budget is set for that.
+template <class F, Gradient G>
+constexpr int group_min_blocks()
+{
+  if constexpr (requires { F::id; }) // built-in objectives
+    return F::id == obj::kMlp && G == Gradient::Adjoint ? 2 : 1;
+  else
+    return 1;
+}
+
 template <class F, int D, int NT, Gradient G, bool HREG, class Prov>
-__global__ __launch_bounds__(NT) void
+__global__ __launch_bounds__(NT, (group_min_blocks<F, G>())) void
 bfgs_group_kernel(ClassArgs a, Prov prov)
 {
   __shared__ GroupVectors<D, NT> L;
