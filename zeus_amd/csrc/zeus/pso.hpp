@@ -248,6 +248,26 @@ pso_coord_load(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d)
   return c;
 }
 
+// Initial position and velocity of one coordinate from its two uniforms
+// (reference pso.cuh initKernel).
+ZEUS_D ZEUS_FI void
+pso_init_coord(const ProblemDesc& P, double u0, double u1, double& x, double& v)
+{
+  const double vr = (P.upper - P.lower) * 0.1;
+  x = P.lower + (P.upper + (-P.lower)) * u0;
+  v = -vr + (vr + (-(-vr))) * u1;
+}
+
+// New velocity of one coordinate (reference pso.cuh iterKernel); the new
+// position is xi + the result.
+ZEUS_D ZEUS_FI double
+pso_velocity(double xi, double vi, double pb, double g, double u0, double u1)
+{
+  const double r1 = 0.0 + (1.0 + (-0.0)) * u0;
+  const double r2 = 0.0 + (1.0 + (-0.0)) * u1;
+  return kPsoW * vi + kPsoC1 * r1 * (pb - xi) + kPsoC2 * r2 * (g - xi);
+}
+
 // pso_coord from preloaded operands: the same arithmetic, same order.
 ZEUS_D ZEUS_FI double
 pso_coord_apply(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d,
@@ -255,18 +275,15 @@ pso_coord_apply(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d,
 {
   const int64_t o = s * D + d;
   if (a.iter == 0) {
-    const double vr = (P.upper - P.lower) * 0.1;
-    const double rx = P.lower + (P.upper + (-P.lower)) * c.u0;
-    const double rv = -vr + (vr + (-(-vr))) * c.u1;
+    double rx, rv;
+    pso_init_coord(P, c.u0, c.u1, rx, rv);
     a.X[o] = rx;
     a.V[o] = rv;
     a.pbx[o] = rx;
     return rx;
   }
-  const double r1 = 0.0 + (1.0 + (-0.0)) * c.u0;
-  const double r2 = 0.0 + (1.0 + (-0.0)) * c.u1;
-  const double xi = c.x, vi = c.v, pb = c.pb;
-  const double nv = kPsoW * vi + kPsoC1 * r1 * (pb - xi) + kPsoC2 * r2 * (gb[d] - xi);
+  const double xi = c.x;
+  const double nv = pso_velocity(xi, c.v, c.pb, gb[d], c.u0, c.u1);
   a.V[o] = nv;
   a.X[o] = xi + nv;
   return xi + nv;
