@@ -72,13 +72,19 @@ typedef int32_t zeus_status_t;
 
 #define ZEUS_FORM_AUTO 0   /* autotuner's choice among all forms          */
 #define ZEUS_FORM_THREAD 1 /* one lane per start (reference sequential)   */
-#define ZEUS_FORM_GROUP 2  /* workgroup forms (reference parallel): one
+#define ZEUS_FORM_GROUP 2  /* cooperative forms (reference parallel): one
                               workgroup per start, or -- Gaussian, adjoint
                               gradient, unless group_threads is set -- the
-                              tile form; the autotuner picks              */
+                              tile form, or -- separable objectives of
+                              small dim, unless group_threads is set --
+                              the team form; the autotuner picks          */
 #define ZEUS_FORM_TILE 3   /* one workgroup per tile of 16 starts, the
                               objective evaluated for the tile on FP64 MFMA
                               (Gaussian with dim >= 16, adjoint gradient)  */
+#define ZEUS_FORM_TEAM 4   /* teams of group_threads = 4, 8 or 16 lanes per
+                              start, 64 / team starts per wavefront
+                              (Rosenbrock, Rastrigin, Ackley, sphere with
+                              small dim); part of ZEUS_FORM_GROUP and AUTO  */
 
 /* Trajectory recording of one problem (reference zeus.cuh:62-103,189-239;
    recording in bfgs_sequential.cuh:100-109 and bfgs_parallel.cuh:155-184).

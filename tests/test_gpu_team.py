@@ -1,0 +1,83 @@
+"""GPU parity of the team form (a few lanes per start, several starts per
+wavefront; separable objectives of small dimension) against the CPU oracle
+and against the per-workgroup form, through the C ABI.
+
+Sums run in another fixed order than the serial oracle, so parity is to
+tolerance, as for the other cooperative forms: statuses agree (a line-search
+decision taken at roundoff level may send a start elsewhere, so a small
+share may differ), and where both converge the minima agree to
+|df| <= 1e-8 max(1, |f|) and the minimizers to 1e-6."""
+import numpy as np
+import pytest
+
+import zeus_amd as z
+
+pytestmark = pytest.mark.gpu
+F_TOL, X_TOL = 1e-8, 1e-6
+
+CASES = [  # objective, oracle id, dim, box, tol, MAX_ITER, team sizes offered
+    ("rosenbrock", lambda: z.Rosenbrock(10), 10, 5.0, 1e-8, 10000, (4, 8)),
+    ("rastrigin", lambda: z.Rastrigin(20), 20, 5.12, 1e-5, 500, (8, 16)),
+    ("ackley", lambda: z.Ackley(10), 10, 32.768, 1e-5, 500, (4, 8)),
+]
+
+
+def close_f(a, b):
+    return abs(a - b) <= F_TOL * max(1.0, abs(a), abs(b))
+
+
+def _oracle_id(orc, name):
+    return {"rosenbrock": orc.ROSENBROCK, "rastrigin": orc.RASTRIGIN, "ackley": orc.ACKLEY}[name]
+
+
+@pytest.mark.parametrize("grad", ["adjoint", "dual"])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_team_form_matches_oracle(orc, case, grad):
+    name, make, dim, box, tol, max_iter, sizes = case
+    N = 256
+    o = orc.run(_oracle_id(orc, name), dim, -box, box, N, max_iter, 10, N + 1, tol, 42)
+    for ts in sizes:
+        r = z.Zeus(make(), -box, box, N, max_iter, 10, N + 1, name, tol, 42, 0, form="team",
+                   group_threads=ts, gradient=grad, per_start=True)
+        ps = r.per_start
+        assert (ps["status"] == o.status).mean() >= 0.95, (name, ts)
+        both = (ps["status"] == 1) & (o.status == 1)
+        same = np.array([close_f(a, b) for a, b in zip(ps["fval"], o.fval)])
+        assert (both & same).sum() >= 0.9 * both.sum() and both.sum() > 0, (name, ts)
+        assert np.abs(ps["x"][both & same] - o.x[both & same]).max() <= X_TOL, (name, ts)
+        assert r.sum_grads == int(ps["grads"].sum()) and r.sum_iter == int(ps["iter"].sum())
+
+
+def test_team_form_agrees_with_group_form():
+    """Same PSO result, same method: the team and per-workgroup forms end in the
+    same place for almost every start."""
+    N = 512
+    f = z.Rastrigin(20)
+    a = z.Zeus(f, -5.12, 5.12, N, 300, 10, N + 1, "r", 1e-5, 7, 0, form="team",
+               group_threads=16, per_start=True)
+    b = z.Zeus(f, -5.12, 5.12, N, 300, 10, N + 1, "r", 1e-5, 7, 0, form="group",
+               group_threads=64, per_start=True)
+    pa, pb = a.per_start, b.per_start
+    assert (pa["status"] == pb["status"]).mean() >= 0.95
+    same = np.array([close_f(x, y) for x, y in zip(pa["fval"], pb["fval"])])
+    assert same.mean() >= 0.9
+
+
+def test_team_form_early_stop_and_counts():
+    N = 1000
+    r = z.Zeus(z.Rosenbrock(10), -5, 5, N, 10000, 10, 20, "r", 1e-8, 42, 0, form="team",
+               group_threads=8, per_start=True)
+    st = r.per_start["status"]
+    assert r.c.claimed == int((st == 1).sum()) >= 20
+    assert r.c.stopped == int((st == 2).sum())
+    assert r.c.claimed + r.c.stopped + r.c.surrendered == N
+    assert r.status == 1 and abs(r.fval) <= 1e-12
+
+
+def test_autotuner_offers_the_team_form():
+    f = z.Ackley(10)
+    for _ in range(2):
+        r = z.Zeus(f, -32.768, 32.768, 2048, 200, 10, 2049, "a", 1e-5, 3, 0)
+    form, threads = z.tuned_config(3, 10, 0)
+    assert form in (z.capi.FORM_THREAD, z.capi.FORM_GROUP, z.capi.FORM_TEAM)
+    assert r.status in (0, 1)

@@ -257,7 +257,8 @@ def _options(gradient="adjoint", form=None, parallel=True, group_threads=0, nzer
         form = capi.FORM_GROUP if parallel else capi.FORM_THREAD
     elif isinstance(form, str):
         form = {"auto": capi.FORM_AUTO, "thread": capi.FORM_THREAD,
-                "group": capi.FORM_GROUP, "tile": capi.FORM_TILE}[form]
+                "group": capi.FORM_GROUP, "tile": capi.FORM_TILE,
+                "team": capi.FORM_TEAM}[form]
     return capi.Options(g, int(form), int(group_threads), int(nzerosteps),
                         1 if use_graph else 0, capi.FLAG_METRICS if metrics else 0)
 

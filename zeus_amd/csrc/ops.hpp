@@ -95,6 +95,71 @@ struct Ops {
     return hipGetLastError();
   }
 
+  // team form: separable built-in objectives whose H rows fit a lane
+  static constexpr bool kTeamObj = F::id == obj::kRosenbrock || F::id == obj::kRastrigin ||
+                                   F::id == obj::kAckley || F::id == obj::kSphere;
+  template <int TS>
+  static constexpr bool kTeamOk = kTeamObj && dev::team_fits<D, TS>();
+
+  template <int TS, Gradient G>
+  static auto team_kernel()
+  {
+    return &dev::bfgs_team_kernel<F, D, TS, G, dev::BuiltinProvider>;
+  }
+  template <int TS>
+  static hipError_t team_ts(const dev::ClassArgs& a, int gradient, int grid, hipStream_t s)
+  {
+    if constexpr (!kTeamOk<TS>) {
+      return hipErrorInvalidConfiguration;
+    } else {
+      if (gradient == 1)
+        team_kernel<TS, Gradient::Dual>()<<<grid, dev::kTeamThreads, 0, s>>>(a, {});
+      else
+        team_kernel<TS, Gradient::Adjoint>()<<<grid, dev::kTeamThreads, 0, s>>>(a, {});
+      return hipGetLastError();
+    }
+  }
+  static hipError_t bfgs_team(const dev::ClassArgs& a, int gradient, int ts, int grid,
+                              hipStream_t s)
+  {
+    if (fault_take_launch())
+      return hipErrorLaunchFailure;
+    if (ts == 4)
+      return team_ts<4>(a, gradient, grid, s);
+    if (ts == 8)
+      return team_ts<8>(a, gradient, grid, s);
+    if (ts == 16)
+      return team_ts<16>(a, gradient, grid, s);
+    return hipErrorInvalidConfiguration;
+  }
+  template <int TS>
+  static int team_occ(int gradient)
+  {
+    if constexpr (!kTeamOk<TS>) {
+      return 1;
+    } else {
+      static int cached[2] = {0, 0}; // per gradient; queried once per process
+      int& c = cached[gradient == 1];
+      if (c == 0) {
+        int n = 0;
+        const hipError_t e =
+          gradient == 1
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void*>(team_kernel<TS, Gradient::Dual>()),
+                dev::kTeamThreads, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void*>(team_kernel<TS, Gradient::Adjoint>()),
+                dev::kTeamThreads, 0);
+        c = (e == hipSuccess && n > 0) ? n : 1;
+      }
+      return c;
+    }
+  }
+  static int team_blocks_per_cu(int gradient, int ts)
+  {
+    return ts == 4 ? team_occ<4>(gradient) : ts == 8 ? team_occ<8>(gradient) : team_occ<16>(gradient);
+  }
+
   static hipError_t bfgs_group(const dev::ClassArgs& a, int gradient, int nt, int grid,
                                hipStream_t s)
   {
@@ -226,6 +291,14 @@ struct Ops {
       o.group_blocks_per_cu = &group_blocks_per_cu;
       o.group_slab_per_block = &group_slab_per_block;
       o.eval = &eval;
+      if constexpr (kTeamOk<4>)
+        o.team_ts[o.n_team++] = 4;
+      if constexpr (kTeamOk<8>)
+        o.team_ts[o.n_team++] = 8;
+      if constexpr (kTeamOk<16>)
+        o.team_ts[o.n_team++] = 16;
+      o.bfgs_team = &bfgs_team;
+      o.team_blocks_per_cu = &team_blocks_per_cu;
       o.tile_ok = kTileOk;
       o.bfgs_tile = &bfgs_tile;
       o.tile_blocks_per_cu = &tile_blocks_per_cu;

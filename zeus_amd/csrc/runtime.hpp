@@ -3,6 +3,7 @@
 
 #include "zeus/batch.hpp"
 #include "zeus/bfgs_group.hpp"
+#include "zeus/bfgs_team.hpp"
 #include "zeus/bfgs_thread.hpp"
 #include "zeus/bfgs_tile.hpp"
 #include "zeus/prng.hpp"
@@ -29,6 +30,12 @@ struct ClassOps {
                            hipStream_t) = nullptr;
   int (*group_blocks_per_cu)(int gradient, int nt) = nullptr;
   int64_t (*group_slab_per_block)(int nt) = nullptr; // doubles, 0 if resident
+  // team form (several lanes per start, several starts per wavefront)
+  int n_team = 0;
+  int team_ts[3] = {0, 0, 0}; // offered team sizes
+  hipError_t (*bfgs_team)(const dev::ClassArgs&, int gradient, int ts, int grid,
+                          hipStream_t) = nullptr;
+  int (*team_blocks_per_cu)(int gradient, int ts) = nullptr;
   // tile form (16 starts per workgroup, objective on MFMA; adjoint only)
   bool tile_ok = false;
   hipError_t (*bfgs_tile)(const dev::TileArgs&, int grid, hipStream_t) = nullptr;

@@ -310,6 +310,13 @@ candidates(const ClassOps* ops, const zeus_options& opt)
     v.push_back({ZEUS_FORM_THREAD, 0});
   if (want_group && tile_ok && opt.group_threads <= 0)
     v.push_back({ZEUS_FORM_TILE, dev::kTileThreads});
+  if ((want_group || opt.form == ZEUS_FORM_TEAM) && opt.group_threads <= 0)
+    for (int k = 0; k < ops->n_team; ++k)
+      v.push_back({ZEUS_FORM_TEAM, ops->team_ts[k]});
+  if (opt.form == ZEUS_FORM_TEAM && opt.group_threads > 0)
+    for (int k = 0; k < ops->n_team; ++k)
+      if (ops->team_ts[k] == opt.group_threads)
+        v.push_back({ZEUS_FORM_TEAM, ops->team_ts[k]});
   if (want_group || v.empty())
     for (int k = 0; k < ops->n_group; ++k)
       if (opt.group_threads <= 0 || ops->group_nt[k] == opt.group_threads)
@@ -462,6 +469,12 @@ group_grid(const ClassPlan& c, const TuneVal& v, int gradient)
     const int g = int(std::min<int64_t>(c.pso_blocks, int64_t(g_ds.n_cu) * occ));
     return std::max(g, 1);
   }
+  if (v.form == ZEUS_FORM_TEAM) {
+    const int occ = c.ops->team_blocks_per_cu(gradient, v.threads);
+    const int64_t teams = dev::kTeamThreads / v.threads;
+    const int g = int(std::min<int64_t>((c.total + teams - 1) / teams, int64_t(g_ds.n_cu) * occ));
+    return std::max(g, 1);
+  }
   const int occ = c.ops->group_blocks_per_cu(gradient, v.threads);
   const int g = int(std::min<int64_t>(c.total, int64_t(g_ds.n_cu) * occ));
   return std::max(g, 1);
@@ -474,6 +487,8 @@ slab_doubles(const ClassPlan& c, const TuneVal& v, int gradient)
     return c.ops->thread_slab_per_start * c.total;
   if (v.form == ZEUS_FORM_TILE)
     return c.ops->tile_slot_doubles * dev::kTileStarts * group_grid(c, v, gradient);
+  if (v.form == ZEUS_FORM_TEAM)
+    return 0; // H in registers
   return c.ops->group_slab_per_block(v.threads) * group_grid(c, v, gradient);
 }
 
@@ -491,6 +506,8 @@ launch_bfgs_cfg(ClassPlan& c, const TuneVal& v, Control* ctl_all, int gradient, 
                            c.h_slab};
     return c.ops->bfgs_tile(ta, group_grid(c, v, gradient), st);
   }
+  if (v.form == ZEUS_FORM_TEAM)
+    return c.ops->bfgs_team(a, gradient, v.threads, group_grid(c, v, gradient), st);
   a.h_slab_stride = c.ops->group_slab_per_block(v.threads);
   return c.ops->bfgs_group(a, gradient, v.threads, group_grid(c, v, gradient), st);
 }
@@ -631,13 +648,13 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     c.key = TuneKey{c.ops->objective, c.ops->dim, opt.gradient, size_bucket(c.total)};
     c.cands = candidates(c.ops, opt);
     if (c.traced) {
-      // recording runs one shape, not the tile form, and is not timed
-      // against others (each trial would rewrite the trajectory)
-      if (opt.form == ZEUS_FORM_TILE)
+      // recording runs one shape, not the tile or team form, and is not
+      // timed against others (each trial would rewrite the trajectory)
+      if (opt.form == ZEUS_FORM_TILE || opt.form == ZEUS_FORM_TEAM)
         return ZEUS_E_ARG;
       std::vector<TuneVal> keep;
       for (const auto& v : c.cands)
-        if (v.form != ZEUS_FORM_TILE)
+        if (v.form != ZEUS_FORM_TILE && v.form != ZEUS_FORM_TEAM)
           keep.push_back(v);
       if (keep.empty())
         keep.push_back({ZEUS_FORM_GROUP, c.ops->group_nt[c.ops->n_group - 1]});
