@@ -221,7 +221,7 @@ bfgs_team_kernel(ClassArgs a, Prov prov)
     decltype(auto) f = prov.template get<F>(P);
     const int64_t sl = s - P.start0;
 
-    double x[KC], g[KC], p[KC], xt[KC], gn[KC];
+    double x[KC], g[KC], p[KC], gn[KC]; // the trial point lives in LDS (xl)
     double H[KC][D];
     // the rank from an opaque copy, so that the identity rows below are
     // formed here for each start instead of being kept (spilled) across
@@ -251,6 +251,16 @@ bfgs_team_kernel(ClassArgs a, Prov prov)
         const int i = r + TS * k;
         if (i < D)
           dst[i] = v[k];
+      }
+      team_exchange();
+    };
+    // trial point x + alpha p into the team's LDS slot
+    auto stage_trial = [&](double alpha) {
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        const int i = r + TS * k;
+        if (i < D)
+          xl[i] = ::fma(alpha, p[k], x[k]);
       }
       team_exchange();
     };
@@ -287,10 +297,7 @@ bfgs_team_kernel(ClassArgs a, Prov prov)
       double alpha = 1.0, f1 = 0.0;
       bool accepted = false;
       for (int t = 0; t < kLineSearchTrials; ++t) {
-#pragma unroll
-        for (int k = 0; k < KC; ++k)
-          xt[k] = ::fma(alpha, p[k], x[k]);
-        stage(xt, xl);
+        stage_trial(alpha);
         f1 = E::value(xl, r);
         ++ls_evals;
         if (f1 <= ::fma(kArmijoC1 * alpha, ddir, bestVal) && f1 < bestVal) {
@@ -302,10 +309,7 @@ bfgs_team_kernel(ClassArgs a, Prov prov)
       }
       double fnew = f1;
       if (!accepted) {
-#pragma unroll
-        for (int k = 0; k < KC; ++k)
-          xt[k] = ::fma(alpha, p[k], x[k]);
-        stage(xt, xl);
+        stage_trial(alpha);
         fnew = E::value(xl, r);
         ++ls_evals;
       }
@@ -328,7 +332,8 @@ bfgs_team_kernel(ClassArgs a, Prov prov)
       double t0 = 0.0, t1 = 0.0;
 #pragma unroll
       for (int k = 0; k < KC; ++k) {
-        sc[k] = xt[k] - x[k];
+        const int i = r + TS * k;
+        sc[k] = i < D ? xl[i] - x[k] : 0.0;
         const double y = gn[k] - g[k];
         t0 = ::fma(sc[k], y, t0);
         t1 = ::fma(y, v[k] + p[k], t1);
@@ -369,7 +374,7 @@ bfgs_team_kernel(ClassArgs a, Prov prov)
         bestVal = fnew;
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
-          x[k] = xt[k];
+          x[k] = (r + TS * k) < D ? xl[r + TS * k] : 0.0;
           g[k] = gn[k];
         }
       }
