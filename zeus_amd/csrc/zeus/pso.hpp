@@ -295,6 +295,14 @@ pso_coord(const PsoArgs& a, const ProblemDesc& P, int D, int64_t s, int d, const
   return pso_coord_apply(a, P, D, s, d, pso_coord_load(a, P, D, s, d), gb);
 }
 
+// Problem of this workgroup.  A class of one problem (the common case)
+// needs no table lookup, so the problem's descriptor is the first load.
+ZEUS_D ZEUS_FI int
+pso_block_problem(const PsoArgs& a)
+{
+  return a.n_probs == 1 ? 0 : a.block_problem[blockIdx.x];
+}
+
 // Publish the workgroup's best particle into this iteration's partial slot
 // (value and index; the caller copies the coordinates).
 ZEUS_D ZEUS_FI void
@@ -314,7 +322,7 @@ pso_lane_kernel(PsoArgs a, Prov prov)
   __shared__ double sv[4];
   __shared__ int64_t si[4];
   __shared__ int64_t s_best;
-  const int pi = a.block_problem[blockIdx.x];
+  const int pi = pso_block_problem(a);
   const ProblemDesc P = a.probs[pi];
   if (a.iter > P.pso_iter)
     return; // this problem runs fewer iterations (whole workgroup returns)
@@ -370,7 +378,7 @@ pso_group_kernel(PsoArgs a, Prov prov)
   __shared__ double stash[D];
   __shared__ double scalar[4];
   __shared__ int64_t s_best;
-  const int pi = a.block_problem[blockIdx.x];
+  const int pi = pso_block_problem(a);
   const ProblemDesc P = a.probs[pi];
   if (a.iter > P.pso_iter)
     return;
@@ -441,7 +449,7 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   __shared__ int improved[kPsoTile];
   __shared__ int best_j;
 
-  const int pi = a.block_problem[blockIdx.x];
+  const int pi = pso_block_problem(a);
   const ProblemDesc P = a.probs[pi];
   if (a.iter > P.pso_iter)
     return;
