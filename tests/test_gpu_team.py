@@ -81,3 +81,22 @@ def test_autotuner_offers_the_team_form():
     form, threads = z.tuned_config(3, 10, 0)
     assert form in (z.capi.FORM_THREAD, z.capi.FORM_GROUP, z.capi.FORM_TEAM)
     assert r.status in (0, 1)
+
+
+def test_team_form_batch_equals_single_calls_and_edge_shapes():
+    """Starts do not interact without an early stop, so a batch of problems on
+    the team form gives every start what its own call gives, bit for bit;
+    ragged start counts (not a multiple of the teams of a workgroup) and
+    MAX_ITER = 0 take the same paths."""
+    probs = [z.Problem(z.Rastrigin(20), -5.12, 5.12, n, 150, 5, n + 1, 1e-5, 11, stream=i)
+             for i, n in enumerate((37, 129, 5))]
+    probs.append(z.Problem(z.Ackley(10), -32.768, 32.768, 300, 150, 5, 301, 1e-5, 11, stream=9))
+    batch = z.run_batch(probs, form="team", group_threads=8, per_start=True)
+    for p, rb in zip(probs, batch):
+        (rs,) = z.run_batch([p], form="team", group_threads=8, per_start=True)
+        for key in ("status", "iter", "fval", "x", "ls_evals", "grads"):
+            assert np.array_equal(rb.per_start[key], rs.per_start[key]), key
+    (r0,) = z.run_batch([z.Problem(z.Rosenbrock(10), -5, 5, 33, 0, 3, 34, 1e-8, 1)],
+                        form="team", group_threads=4, per_start=True)
+    assert set(r0.per_start["status"]) == {0} and set(r0.per_start["iter"]) == {0}
+    assert r0.c.surrendered == 33
