@@ -2,17 +2,18 @@
 build one workload, run it warm, print one JSON line with its timing and
 counts.
 
-    python -m zeus_amd.workloads c1|c3|c4|c5 [--calls K] [--max-iter M] [--gradient G]
+    python -m zeus_amd.workloads rosenbrock|dijet|mlp|mixed [--calls K] [--max-iter M]
+                                 [--gradient G]
 
-c1  Rosenbrock D = 2, box [-5, 5], 1024 starts, PSO 10, MAX_ITER 10000,
+rosenbrock  Rosenbrock D = 2, box [-5, 5], 1024 starts, PSO 10, MAX_ITER 10000,
     required 100, tol 1e-8, seed 42 (the reference's CPU-runnable case).
-c3  1024 dijet fits x 256 starts: Poisson NLL over 100 bins, D = 4, box
+dijet  1024 dijet fits x 256 starts: Poisson NLL over 100 bins, D = 4, box
     [0, 10], PSO 10, required 100 per fit, tol 1e-8; counts drawn per fit
     from the seed (make_dijet).  MAX_ITER defaults to 100: at tol 1e-8 the
     NLL gradient sits on its roundoff floor and starts surrender.
-c4  MLP 1-16-16-1 tanh (321 parameters) least squares over 256 points, box
+mlp  MLP 1-16-16-1 tanh (321 parameters) least squares over 256 points, box
     [-2, 2], 2048 starts, PSO 10, tol 1e-8; MAX_ITER defaults to 200.
-c5  64 problems in one batched call: Rosenbrock D in {2, 10, 50}, Rastrigin
+mixed  64 problems in one batched call: Rosenbrock D in {2, 10, 50}, Rastrigin
     D = 20, Ackley D = 10, Gaussian D in {32, 150} (C = A A^T / D + I), start
     counts log-uniform in [128, 4096], required min(100, starts); kinds and
     counts drawn from a numpy generator seeded with 42; PSO 10, MAX_ITER
@@ -34,7 +35,7 @@ import numpy as np
 import zeus_amd as z
 
 
-def _c5_problems():
+def _mixed_problems():
     rng = np.random.default_rng(42)
     probs, names = [], []
     for i in range(64):
@@ -64,12 +65,12 @@ def _c5_problems():
 
 def run(name: str, calls: int, max_iter: int | None, gradient: str) -> dict:
     out = {"workload": name, "gradient": gradient, "data": "synthetic", "dtype": "f64"}
-    if name == "c1":
+    if name == "rosenbrock":
         def call():
             return [z.Zeus(z.Rosenbrock(2), -5, 5, 1024, max_iter or 10000, 10, 100,
                            "rosenbrock", 1e-8, 42, 0, form="auto", gradient=gradient)]
         out["starts"] = 1024
-    elif name == "c3":
+    elif name == "dijet":
         it = max_iter or 100
         probs = [z.Problem(z.LogLikelihood.from_params(z.make_dijet(42, e)), 0.0, 10.0, 256,
                            it, 10, 100, 1e-8, 42, stream=e) for e in range(1024)]
@@ -77,7 +78,7 @@ def run(name: str, calls: int, max_iter: int | None, gradient: str) -> dict:
         def call():
             return z.run_batch(probs, gradient=gradient)
         out.update(starts=1024 * 256, max_iter=it)
-    elif name == "c4":
+    elif name == "mlp":
         it = max_iter or 200
         d = z.make_mlp_data(42, 256)
         f = z.Mlp(d[:256], d[256:])
@@ -85,14 +86,14 @@ def run(name: str, calls: int, max_iter: int | None, gradient: str) -> dict:
         def call():
             return [z.Zeus(f, -2, 2, 2048, it, 10, 100, "mlp", 1e-8, 42, 0, gradient=gradient)]
         out.update(starts=2048, max_iter=it)
-    elif name == "c5":
-        probs, names = _c5_problems()
+    elif name == "mixed":
+        probs, names = _mixed_problems()
 
         def call():
             return z.run_batch(probs, gradient=gradient)
         out.update(starts=sum(p.N for p in probs), problems=len(probs))
     else:
-        raise SystemExit(f"unknown workload {name!r} (c1, c3, c4, c5)")
+        raise SystemExit(f"unknown workload {name!r} (rosenbrock, dijet, mlp, mixed)")
     call()  # warm-up
     best, res = float("inf"), None
     for _ in range(calls):
@@ -107,16 +108,16 @@ def run(name: str, calls: int, max_iter: int | None, gradient: str) -> dict:
                sum_iter=sum(r.sum_iter for r in res), sum_ls_evals=sum(r.sum_ls_evals for r in res),
                sum_grads=sum(r.sum_grads for r in res), ms_pso=res[0].ms_pso,
                ms_opt=res[0].ms_opt)
-    if name == "c3":  # the work unit is one bin evaluation (100 per objective evaluation)
+    if name == "dijet":  # the work unit is one bin evaluation (100 per objective evaluation)
         out["bin_evals_per_s"] = evals * 100 / best
-    if name == "c4":
+    if name == "mlp":
         out["start_iterations_per_s"] = out["sum_iter"] / best
     return out
 
 
 def main(argv: list[str]) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("workload", choices=["c1", "c3", "c4", "c5"])
+    ap.add_argument("workload", choices=["rosenbrock", "dijet", "mlp", "mixed"])
     ap.add_argument("--calls", type=int, default=2)
     ap.add_argument("--max-iter", type=int, default=None)
     ap.add_argument("--gradient", default="adjoint", choices=["adjoint", "dual"])
