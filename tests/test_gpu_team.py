@@ -100,3 +100,27 @@ def test_team_form_batch_equals_single_calls_and_edge_shapes():
                         form="team", group_threads=4, per_start=True)
     assert set(r0.per_start["status"]) == {0} and set(r0.per_start["iter"]) == {0}
     assert r0.c.surrendered == 33
+
+
+def test_settled_batch_runs_classes_side_by_side():
+    """After the first call settles every class's shape, a batch's classes run
+    on side streams (forked from and joined into the caller's stream), eagerly
+    or as a captured graph: the results are those of the first call, bit for
+    bit (classes share nothing but the stream order)."""
+    C = z.make_spd(42, 3, 32)
+    probs = [
+        z.Problem(z.Rastrigin(20), -5.12, 5.12, 200, 300, 5, 201, 1e-5, 3, stream=1),
+        z.Problem(z.Ackley(10), -32.768, 32.768, 150, 300, 5, 151, 1e-5, 3, stream=2),
+        z.Problem(z.Gaussian(C), -5, 5, 100, 1000, 5, 101, 1e-8, 3, stream=3),
+        z.Problem(z.Rosenbrock(2), -5, 5, 64, 1000, 5, 65, 1e-8, 3, stream=4),
+    ]
+    first = z.run_batch(probs, per_start=True)
+    runs = [z.run_batch(probs, per_start=True),
+            z.run_batch(probs, per_start=True, use_graph=True),
+            z.run_batch(probs, per_start=True, use_graph=True)]
+    for later in runs:
+        for a, b in zip(first, later):
+            for key in ("status", "iter", "fval", "x"):
+                assert np.array_equal(a.per_start[key], b.per_start[key]), key
+            assert a.idx == b.idx and a.fval == b.fval
+    assert runs[0][0].ms_pso > 0 and runs[0][0].ms_opt > 0

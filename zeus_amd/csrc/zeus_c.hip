@@ -90,6 +90,13 @@ struct DeviceState {
   double tick_hz = 1e8; // rate of the device's constant clock (wall_clock64)
   hipStream_t stream = nullptr;
   hipEvent_t ev[6] = {};
+  // classes of a settled batch run side by side on these streams (default
+  // priority), forked from and joined back into the caller's stream
+  static constexpr int kSide = 4;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kSide] = {};
+  std::vector<hipEvent_t> cls_ev; // per class: PSO start, PSO end, BFGS end
   void* arena = nullptr;
   size_t arena_bytes = 0;
   void* pinned = nullptr; // host mirror of the arena's upload/download regions
@@ -152,6 +159,14 @@ ensure_device()
     return ZEUS_E_LAUNCH;
   for (auto& e : g_ds.ev)
     if (hipEventCreate(&e) != hipSuccess)
+      return ZEUS_E_LAUNCH;
+  for (auto& sd : g_ds.side)
+    if (hipStreamCreateWithFlags(&sd, hipStreamNonBlocking) != hipSuccess)
+      return ZEUS_E_LAUNCH;
+  if (hipEventCreateWithFlags(&g_ds.fork, hipEventDisableTiming) != hipSuccess)
+    return ZEUS_E_LAUNCH;
+  for (auto& e : g_ds.join)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
       return ZEUS_E_LAUNCH;
   g_ds.init = true;
   return ZEUS_OK;
@@ -757,6 +772,31 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   // captured once per batch shape and replayed as a hipGraph; the problem
   // descriptors are copied before it, outside the graph.  Fills are kernels
   // (no memset nodes), so replays start from cleared control blocks.
+  // Classes run side by side once every class's shape is settled (no
+  // autotuner timing in this call): class k on side stream slot[k], the
+  // heaviest classes (starts x dimension) spread first.  A class of surrendering
+  // starts can keep a few wavefronts busy for a long time; the other classes
+  // fill the rest of the GPU meanwhile.
+  bool concurrent = classes.size() > 1;
+  for (const auto& c : classes)
+    concurrent = concurrent && c.cands.size() == 1;
+  std::vector<int> slot(classes.size(), 0);
+  if (concurrent) {
+    std::vector<size_t> order(classes.size());
+    for (size_t k = 0; k < order.size(); ++k)
+      order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+      return classes[x].total * classes[x].ops->dim > classes[y].total * classes[y].ops->dim;
+    });
+    for (size_t r = 0; r < order.size(); ++r)
+      slot[order[r]] = int(r % DeviceState::kSide);
+    while (g_ds.cls_ev.size() < 3 * classes.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess)
+        return fail(hipErrorOutOfMemory);
+      g_ds.cls_ev.push_back(e);
+    }
+  }
   auto sequence = [&](bool timed) -> hipError_t {
     hipError_t err = hipSuccess;
     dev::init_controls_kernel<<<(ctl_total + 255) / 256, 256, 0, st>>>(d_ctl, ctl_total,
@@ -767,6 +807,51 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     }
     if ((err = hipGetLastError()) != hipSuccess)
       return err;
+    if (concurrent) {
+      if (timed)
+        (void)hipEventRecord(g_ds.ev[0], st);
+      if ((err = hipEventRecord(g_ds.fork, st)) != hipSuccess)
+        return err;
+      bool used[DeviceState::kSide] = {};
+      for (size_t k = 0; k < classes.size(); ++k) {
+        auto& c = classes[k];
+        hipStream_t cs = g_ds.side[slot[k]];
+        if (!used[slot[k]] && (err = hipStreamWaitEvent(cs, g_ds.fork, 0)) != hipSuccess)
+          return err;
+        used[slot[k]] = true;
+        if (timed)
+          (void)hipEventRecord(g_ds.cls_ev[3 * k], cs);
+        if ((err = launch_class(c, d_ctl, opt.gradient, cs)) != hipSuccess)
+          return err;
+        for (const auto& d : c.desc)
+          if (d.traj.fval) {
+            const int64_t n = int64_t(d.traj.steps) * d.traj.starts * d.dim;
+            const int g = int(std::min<int64_t>((n + 255) / 256, 4096));
+            dev::traj_fill_kernel<<<g, 256, 0, cs>>>(d.traj, d.dim);
+          }
+        if (timed)
+          (void)hipEventRecord(g_ds.cls_ev[3 * k + 1], cs);
+        if ((err = launch_bfgs(c, d_ctl, opt.gradient, cs)) != hipSuccess)
+          return err;
+        g_ds.last[{c.ops->objective, c.ops->dim, opt.gradient}] = c.cfg;
+        if (timed)
+          (void)hipEventRecord(g_ds.cls_ev[3 * k + 2], cs);
+      }
+      for (int j = 0; j < DeviceState::kSide; ++j)
+        if (used[j]) {
+          if ((err = hipEventRecord(g_ds.join[j], g_ds.side[j])) != hipSuccess)
+            return err;
+          if ((err = hipStreamWaitEvent(st, g_ds.join[j], 0)) != hipSuccess)
+            return err;
+        }
+      if (timed)
+        (void)hipEventRecord(g_ds.ev[2], st);
+      for (auto& c : classes) {
+        dev::summary_kernel<<<int(c.probs.size()), dev::kSummaryThreads, 0, st>>>(
+          c.d_desc, d_ctl + c.ctl0, c.out, d_bx, d_bxo + c.ctl0);
+      }
+      return hipGetLastError();
+    }
     if (timed)
       (void)hipEventRecord(g_ds.ev[0], st);
     for (auto& c : classes)
@@ -815,6 +900,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       key.push_back(uint64_t(c.cfg.form) << 32 | uint32_t(c.cfg.threads));
       key.push_back(uint64_t(c.xorwow));
     }
+    key.push_back(uint64_t(concurrent));
     hipGraphExec_t exec = nullptr;
     for (const auto& ge : g_ds.graphs)
       if (ge.key == key)
@@ -856,6 +942,14 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   float ms_pso = 0.f, ms_opt = 0.f;
   if (graphed) { // one replay: its device time (PSO, BFGS and summary) in ms_opt
     (void)hipEventElapsedTime(&ms_opt, g_ds.ev[0], g_ds.ev[2]);
+  } else if (concurrent) { // classes side by side: the longest PSO and BFGS phases
+    for (size_t k = 0; k < classes.size(); ++k) {
+      float a = 0.f, b = 0.f;
+      (void)hipEventElapsedTime(&a, g_ds.cls_ev[3 * k], g_ds.cls_ev[3 * k + 1]);
+      (void)hipEventElapsedTime(&b, g_ds.cls_ev[3 * k + 1], g_ds.cls_ev[3 * k + 2]);
+      ms_pso = std::max(ms_pso, a);
+      ms_opt = std::max(ms_opt, b);
+    }
   } else {
     (void)hipEventElapsedTime(&ms_pso, g_ds.ev[0], g_ds.ev[1]);
     (void)hipEventElapsedTime(&ms_opt, g_ds.ev[1], g_ds.ev[2]);
