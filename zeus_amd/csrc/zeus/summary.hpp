@@ -126,26 +126,35 @@ summary_kernel(const ProblemDesc* probs, Control* ctl, StartOut out, double* bes
   const ProblemDesc P = probs[pi];
   double v = __builtin_nan("");
   int64_t idx = int64_t(1) << 62;
+  int my_status = 0, my_iter = 0; // fields of this thread's best start
+  double my_gnorm = 0.0;
   int64_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll 4
   for (int64_t k = threadIdx.x; k < P.n_starts; k += kSummaryThreads) {
     const int64_t s = P.start0 + k;
     const double f = out.fval[s];
+    const int st = out.status[s];
+    const int it = out.iter[s];
+    const double gn = out.gnorm[s];
     if (better(f, k, v, idx)) {
       v = f;
       idx = k;
+      my_status = st;
+      my_iter = it;
+      my_gnorm = gn;
     }
-    const int st = out.status[s];
     cnt[0] += st == kStatusConverged;
     cnt[1] += st == kStatusStopped;
     cnt[2] += (st != kStatusConverged && st != kStatusStopped);
     cnt[3] += st == kStatusNonFinite;
-    cnt[4] += out.iter[s];
+    cnt[4] += it;
     cnt[5] += out.ls_evals[s];
     cnt[6] += out.grads[s];
     cnt[7] += out.near[s];
   }
-  block_argmin<kSummaryThreads>(v, idx, sv, si);
+  const int64_t mine = idx; // this thread's own best before the reduction
+  const double mine_v = v;
+  block_argmin<kSummaryThreads>(v, idx, sv, si); // winner in thread 0
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     int64_t t = cnt[c];
@@ -155,35 +164,48 @@ summary_kernel(const ProblemDesc* probs, Control* ctl, StartOut out, double* bes
     if ((threadIdx.x & 63) == 0)
       sc[threadIdx.x >> 6][c] = t;
   }
-  __syncthreads();
   if (threadIdx.x == 0) {
-    int64_t tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int w = 0; w < NW; ++w)
-      for (int c = 0; c < 8; ++c)
-        tot[c] += sc[w][c];
-    ctl[pi].actual = int32_t(tot[7]);
-    const int64_t b = P.n_starts > 0 ? (idx < P.n_starts ? idx : 0) : -1;
+    // every thread's first candidate is accepted, so with any start at all
+    // the winner is a real start (start 0 when every value is NaN)
+    const int64_t b = P.n_starts > 0 ? idx : -1;
     ctl[pi].best_idx = int32_t(b);
-    ctl[pi].claimed = int32_t(tot[0]);
-    ctl[pi].stopped = int32_t(tot[1]);
-    ctl[pi].surrendered = int32_t(tot[2]);
-    ctl[pi].nonfinite = int32_t(tot[3]);
-    ctl[pi].sum_iter = tot[4];
-    ctl[pi].sum_ls_evals = tot[5];
-    ctl[pi].sum_grads = tot[6];
-    if (b >= 0) {
-      const int64_t s = P.start0 + b;
-      ctl[pi].best_status = out.status[s];
-      ctl[pi].best_iter = out.iter[s];
-      ctl[pi].best_fval = out.fval[s];
-      ctl[pi].best_gnorm = out.gnorm[s];
-    }
     sbest = b;
   }
   __syncthreads();
-  if (sbest >= 0)
+  const int64_t b = sbest;
+  // the thread whose own best is the winner holds its fields: written from
+  // registers, with no dependent reload of the winner's outputs
+  if (b >= 0 && mine == b) {
+    ctl[pi].best_status = my_status;
+    ctl[pi].best_iter = my_iter;
+    ctl[pi].best_fval = mine_v;
+    ctl[pi].best_gnorm = my_gnorm;
+  }
+  if (threadIdx.x < 8) { // one counter per thread, wavefronts in order
+    const int c = threadIdx.x;
+    int64_t tot = 0;
+    for (int w = 0; w < NW; ++w)
+      tot += sc[w][c];
+    if (c == 0)
+      ctl[pi].claimed = int32_t(tot);
+    else if (c == 1)
+      ctl[pi].stopped = int32_t(tot);
+    else if (c == 2)
+      ctl[pi].surrendered = int32_t(tot);
+    else if (c == 3)
+      ctl[pi].nonfinite = int32_t(tot);
+    else if (c == 4)
+      ctl[pi].sum_iter = tot;
+    else if (c == 5)
+      ctl[pi].sum_ls_evals = tot;
+    else if (c == 6)
+      ctl[pi].sum_grads = tot;
+    else
+      ctl[pi].actual = int32_t(tot);
+  }
+  if (b >= 0)
     for (int d = threadIdx.x; d < P.dim; d += kSummaryThreads)
-      best_x[best_x_off[pi] + d] = out.x[(P.start0 + sbest) * P.dim + d];
+      best_x[best_x_off[pi] + d] = out.x[(P.start0 + b) * P.dim + d];
 }
 
 // Objective construction from a problem descriptor (built-in objectives).
