@@ -799,12 +799,20 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   }
   auto sequence = [&](bool timed) -> hipError_t {
     hipError_t err = hipSuccess;
-    dev::init_controls_kernel<<<(ctl_total + 255) / 256, 256, 0, st>>>(d_ctl, ctl_total,
-                                                                       nullptr, 0);
-    for (auto& c : classes) {
+    // the controls and the first class's gBest coordinates in one launch
+    // (grid-stride over the coordinates), the other classes' after it
+    for (size_t i = 0; i < classes.size(); ++i) {
+      ClassPlan& c = classes[i];
       const int64_t ng = int64_t(c.probs.size()) * c.ops->dim;
-      dev::init_controls_kernel<<<int((ng + 255) / 256), 256, 0, st>>>(nullptr, 0, c.gbx, ng);
+      if (i == 0)
+        dev::init_controls_kernel<<<(ctl_total + 255) / 256, 256, 0, st>>>(d_ctl, ctl_total,
+                                                                           c.gbx, ng);
+      else
+        dev::init_controls_kernel<<<int((ng + 255) / 256), 256, 0, st>>>(nullptr, 0, c.gbx, ng);
     }
+    if (classes.empty())
+      dev::init_controls_kernel<<<(ctl_total + 255) / 256, 256, 0, st>>>(d_ctl, ctl_total,
+                                                                         nullptr, 0);
     if ((err = hipGetLastError()) != hipSuccess)
       return err;
     if (concurrent) {
