@@ -267,6 +267,10 @@ def _metrics_dict(m) -> dict:
     return {k: getattr(m, k) for k, _ in capi.Metrics._fields_}
 
 
+# what the library returns for a call without ZEUS_FLAG_METRICS
+_METRICS_ZERO = _metrics_dict(capi.Metrics())
+
+
 def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group_threads=0,
               per_start=False, use_graph=False, metrics=False) -> list[Result]:
     """Many independent problems in one launch sequence (zeus_run_batch)."""
@@ -300,7 +304,7 @@ def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group
     opt = _options(gradient, form, True, group_threads, 0, use_graph, metrics)
     R = (capi.Result * max(n, 1))()
     dims = sum(pr.f.dim for pr in problems)
-    bx = np.zeros(max(dims, 1))
+    bx = (C.c_double * max(dims, 1))()
     so_ptr = None
     arrays = None
     if per_start:
@@ -314,7 +318,7 @@ def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group
             ("gnorm", capi._dp), ("x", capi._dp), ("ls_evals", capi._i32p),
             ("grads", capi._i32p))])
         so_ptr = C.byref(so)
-    rc = lib().zeus_run_batch(P, n, C.byref(opt), R, bx.ctypes.data_as(capi._dp), so_ptr, None)
+    rc = lib().zeus_run_batch(P, n, C.byref(opt), R, bx, so_ptr, None)
     if rc < 0:
         raise capi.ZeusError(rc, "zeus_run_batch")
     out = []
@@ -322,12 +326,14 @@ def run_batch(problems: list[Problem], *, gradient="adjoint", form="auto", group
     for i, pr in enumerate(problems):
         r = R[i]
         res = Result(idx=r.idx, status=r.status, fval=r.fval, gradientNorm=r.gradient_norm,
-                     coordinates=bx[ox:ox + pr.f.dim].copy(), iter=r.iter,
+                     coordinates=np.frombuffer(bx, np.float64, pr.f.dim, 8 * ox).copy(),
+                     iter=r.iter,
                      c=Convergence(r.actual, r.claimed, r.surrendered, r.stopped),
                      ms_opt=r.ms_opt, ms_pso=r.ms_pso, ms_total=r.ms_total,
                      nonfinite=r.nonfinite, sum_iter=r.sum_iter, sum_ls_evals=r.sum_ls_evals,
-                     sum_grads=r.sum_grads, ad=_metrics_dict(r.ad),
-                     bfgs=_metrics_dict(r.bfgs))
+                     sum_grads=r.sum_grads,
+                     ad=_metrics_dict(r.ad) if metrics else dict(_METRICS_ZERO),
+                     bfgs=_metrics_dict(r.bfgs) if metrics else dict(_METRICS_ZERO))
         if rc in (capi.E_ALLOC, capi.E_LAUNCH):
             res.status = capi.MALLOC if rc == capi.E_ALLOC else capi.KERNEL
         if i in trajs and rc == capi.OK:
