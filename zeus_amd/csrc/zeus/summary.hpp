@@ -61,33 +61,35 @@ traj_fill_kernel(TrajOut t, int D)
 static __global__ void
 symmetrize_kernel(const double* C, double* Cs, double* Cf, int D)
 {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e < int64_t(D) * D) {
-    const int i = int(e / D), j = int(e % D);
+  // element indices stay below D * D + (fragments) * 64 < 2^31: 32-bit
+  // division (64-bit division is a long software sequence)
+  const int e = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+  if (e < D * D) {
+    const int i = e / D, j = e - i * D;
     Cs[e] = 0.5 * (C[int64_t(i) * D + j] + C[int64_t(j) * D + i]);
   }
   const int KB = (D + 3) / 4, MB = (D + 15) / 16;
-  if (e < int64_t(MB) * KB * 64) {
-    const int lane = int(e & 63);
-    const int64_t f = e >> 6;
-    const int mb = int(f / KB), kb = int(f % KB);
+  if (e < MB * KB * 64) {
+    const int lane = e & 63;
+    const int f = e >> 6;
+    const int mb = f / KB, kb = f - mb * KB;
     const int r = 16 * mb + (lane & 15), c = 4 * kb + (lane >> 4);
     Cf[e] = (r < D && c < D) ? 0.5 * (C[int64_t(r) * D + c] + C[int64_t(c) * D + r]) : 0.0;
   }
   // upper fragments (cs_product.hpp, CsUpper): row block mb keeps k steps
   // 4 mb .. KB - 1, the diagonal block (k steps 4 mb .. 4 mb + 3) times 0.5
-  int64_t nu = 0;
+  int nu = 0;
   for (int mb = 0; mb < MB; ++mb)
     nu += KB - 4 * mb;
   if (e < nu * 64) {
-    const int lane = int(e & 63);
-    int64_t f = e >> 6;
+    const int lane = e & 63;
+    int f = e >> 6;
     int mb = 0;
     while (f >= KB - 4 * mb) {
       f -= KB - 4 * mb;
       ++mb;
     }
-    const int kb = 4 * mb + int(f);
+    const int kb = 4 * mb + f;
     const int r = 16 * mb + (lane & 15), c = 4 * kb + (lane >> 4);
     const double v =
       (r < D && c < D) ? 0.5 * (C[int64_t(r) * D + c] + C[int64_t(c) * D + r]) : 0.0;
