@@ -109,6 +109,8 @@ def lib():
                               _dp, _dp, _dp, _i64p, _i64p, _dp, C.POINTER(Best)]
         L.orc_pso.restype = C.c_int
         L.orc_pso.argtypes = [C.POINTER(Problem), _dp, _dp, _dp, _dp]
+        L.orc_pso_swarm.restype = C.c_int
+        L.orc_pso_swarm.argtypes = [C.POINTER(Problem), _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_pso_step.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
                                    C.c_double, _dp, _dp, _dp, _dp, _dp]
         L.orc_xorwow_seed.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
@@ -338,3 +340,22 @@ def pso(objective, dim, lower, upper, n_starts, pso_iter, seed, *, stream=0,
     if rc != 0:
         raise RuntimeError(f"orc_pso failed: {rc}")
     return pbx, pbv, gbx, gbv.value
+
+
+def pso_swarm(objective, dim, lower, upper, n_starts, pso_iter, seed, *, stream=0,
+              params=None, prng=PRNG_XORWOW):
+    """The swarm after pso_iter iterations: dict of x, v (final positions and
+    velocities), pbest_x, pbest_val, gbest_x, gbest_val."""
+    params = None if params is None else _f64(params)
+    P = _problem(objective, dim, lower, upper, n_starts, 0, 0, pso_iter, seed,
+                 stream, 0.0, params, None, prng)
+    out = dict(x=np.zeros((n_starts, dim)), v=np.zeros((n_starts, dim)),
+               pbest_x=np.zeros((n_starts, dim)), pbest_val=np.zeros(n_starts),
+               gbest_x=np.zeros(dim))
+    gbv = C.c_double(0.0)
+    rc = lib().orc_pso_swarm(C.byref(P), _ptr(out["x"]), _ptr(out["v"]), _ptr(out["pbest_x"]),
+                             _ptr(out["pbest_val"]), _ptr(out["gbest_x"]), C.byref(gbv))
+    if rc != 0:
+        raise RuntimeError(f"orc_pso_swarm failed: {rc}")
+    out["gbest_val"] = gbv.value
+    return out

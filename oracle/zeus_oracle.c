@@ -959,27 +959,31 @@ pso_gbest(const double* pbv, const double* pbx, int64_t n, int D, double* gbv,
   }
 }
 
+/* The swarm after PSO_ITER iterations: pBest (pbx, pbv), gBest (gbx, gbv)
+   and, when xo / vo are not NULL, the final positions and velocities.
+   Init follows pso.cuh:34-69 (per coordinate: position U(lo, hi], then
+   velocity U(-0.1R, 0.1R], R = hi - lo), each iteration pso.cuh:97-155 (per
+   coordinate r1 then r2, both U(0, 1]; the update is orc_pso_step). */
 static int
 pso_run(const orc_problem* P, int nt, double* pbx, double* pbv, double* gbx,
-        double* gbv)
+        double* gbv, double* xo, double* vo)
 {
   const int64_t N = P->n_starts;
   const int D = P->dim;
   const double lo = P->lower, hi = P->upper;
   const double vr = (hi - lo) * 0.1;
-  const double w = 0.5, c1 = 1.2, c2 = 1.5;
+  const double w = 0.5, c1 = 1.2, c2 = 1.5; /* pso.cuh:302 */
   orc_objective f = {P->objective, D, P->params, P->n_params};
-  double* X = (double*)malloc(sizeof(double) * (size_t)N * D);
-  double* V = (double*)malloc(sizeof(double) * (size_t)N * D);
-  if (!X || !V) {
+  const size_t ND = (size_t)(N > 0 ? N : 1) * (size_t)D;
+  double* X = (double*)malloc(sizeof(double) * ND);
+  double* V = (double*)malloc(sizeof(double) * ND);
+  double* R = (double*)malloc(sizeof(double) * 2 * ND);
+  uint32_t* xs = (uint32_t*)malloc(sizeof(uint32_t) * 6 * ND);
+  if (!X || !V || !R || !xs) {
     free(X);
     free(V);
-    return -2;
-  }
-  uint32_t* xs = (uint32_t*)malloc(sizeof(uint32_t) * 6 * (size_t)(N > 0 ? N : 1) * (size_t)D);
-  if (!xs) {
-    free(X);
-    free(V);
+    free(R);
+    free(xs);
     return -2;
   }
   *gbv = INFINITY;
@@ -1008,25 +1012,29 @@ pso_run(const orc_problem* P, int nt, double* pbx, double* pbv, double* gbx,
     const uint64_t base = 2u * (uint64_t)D * (uint64_t)it;
 #pragma omp parallel for schedule(static) num_threads(nt)
     for (int64_t i = 0; i < N; ++i) {
+      double* r = R + 2 * (size_t)i * D;
       for (int d = 0; d < D; ++d) {
-        size_t o = (size_t)i * D + d;
         double u0, u1;
-        pso_pair(P, (uint64_t)i, base + 2u * (uint64_t)d, xs + 6 * o, &u0, &u1);
-        double r1 = to_range(u0, 0.0, 1.0);
-        double r2 = to_range(u1, 0.0, 1.0);
-        double xi = X[o], vi = V[o], pb = pbx[o], gb = gbx[d];
-        double nv = w * vi + c1 * r1 * (pb - xi) + c2 * r2 * (gb - xi);
-        V[o] = nv;
-        X[o] = xi + nv;
+        pso_pair(P, (uint64_t)i, base + 2u * (uint64_t)d, xs + 6 * ((size_t)i * D + d), &u0,
+                 &u1);
+        r[2 * d] = to_range(u0, 0.0, 1.0);
+        r[2 * d + 1] = to_range(u1, 0.0, 1.0);
       }
-      double fv = orc_value(&f, X + (size_t)i * D);
+      double* Xi = X + (size_t)i * D;
+      orc_pso_step(1, D, w, c1, c2, r, Xi, V + (size_t)i * D, pbx + (size_t)i * D, gbx);
+      double fv = orc_value(&f, Xi);
       if (fv < pbv[i]) {
         pbv[i] = fv;
-        memcpy(pbx + (size_t)i * D, X + (size_t)i * D, sizeof(double) * D);
+        memcpy(pbx + (size_t)i * D, Xi, sizeof(double) * D);
       }
     }
     pso_gbest(pbv, pbx, N, D, gbv, gbx);
   }
+  if (xo)
+    memcpy(xo, X, sizeof(double) * (size_t)N * D);
+  if (vo)
+    memcpy(vo, V, sizeof(double) * (size_t)N * D);
+  free(R);
   free(xs);
   free(X);
   free(V);
@@ -1039,7 +1047,16 @@ orc_pso(const orc_problem* P, double* pbest_x, double* pbest_val,
 {
   if (!P || P->n_starts < 0 || P->dim < 1)
     return -1;
-  return pso_run(P, 0, pbest_x, pbest_val, gbest_x, gbest_val);
+  return pso_run(P, 0, pbest_x, pbest_val, gbest_x, gbest_val, NULL, NULL);
+}
+
+int
+orc_pso_swarm(const orc_problem* P, double* x, double* v, double* pbest_x,
+              double* pbest_val, double* gbest_x, double* gbest_val)
+{
+  if (!P || P->n_starts < 0 || P->dim < 1)
+    return -1;
+  return pso_run(P, 0, pbest_x, pbest_val, gbest_x, gbest_val, x, v);
 }
 
 /* ======================================================================
@@ -1225,7 +1242,7 @@ orc_run(const orc_problem* P, const orc_options* O, int32_t* status,
     double* pbv = (double*)malloc(sizeof(double) * (size_t)N);
     double* gbx = (double*)malloc(sizeof(double) * (size_t)D);
     double gbv;
-    int rc = pso_run(P, nt, pso_x, pbv, gbx, &gbv);
+    int rc = pso_run(P, nt, pso_x, pbv, gbx, &gbv, NULL, NULL);
     free(pbv);
     free(gbx);
     if (rc != 0) {

@@ -170,6 +170,12 @@ int orc_run(const orc_problem* prob, const orc_options* opt, int32_t* status,
 int orc_pso(const orc_problem* prob, double* pbest_x, double* pbest_val,
             double* gbest_x, double* gbest_val);
 
+/* The same, also writing the final positions x and velocities v (n x dim
+   each; NULL to skip). */
+int orc_pso_swarm(const orc_problem* prob, double* x, double* v,
+                  double* pbest_x, double* pbest_val, double* gbest_x,
+                  double* gbest_val);
+
 /* One PSO velocity/position step with explicit coefficients and uniforms
    (used by the reference-KAT tests): r has 2*n*dim entries (r1,r2 per d). */
 void orc_pso_step(int n, int dim, double w, double c1, double c2,

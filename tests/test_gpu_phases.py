@@ -41,6 +41,23 @@ def test_pso_phases_equal_one_call(orc, case):
             assert abs(got["gbest_val"] - o[3]) <= 1e-12 * abs(o[3])
 
 
+@pytest.mark.parametrize("g", GENS, ids=[x.name for x in GENS])
+def test_pso_positions_and_velocities_equal_oracle(orc, g):
+    """Positions and velocities after init and after two steps equal the
+    oracle's swarm bit for bit (lane form, Rosenbrock-3 on [-3, 7]); the
+    oracle's PSO is pinned to the reference rule by a numpy restatement."""
+    N, D, lo, hi = 37, 3, -3.0, 7.0
+    sw = z.Swarm(z.Rosenbrock(D), lo, hi, N, 42, prng=g).init()
+    for T in (0, 1, 2):
+        if T:
+            sw.step(T)
+        got = sw.state()
+        o = orc.pso_swarm(orc.ROSENBROCK, D, lo, hi, N, T, 42, prng=int(g))
+        for k in ("x", "v", "pbest_x", "pbest_val", "gbest_x"):
+            assert np.array_equal(got[k], o[k]), (T, k)
+        assert got["gbest_val"] == o["gbest_val"]
+
+
 def test_pso_step_keeps_invariants():
     sw = z.Swarm(z.Rastrigin(2), -5.12, 5.12, 64, 7).init()
     prev = sw.state()

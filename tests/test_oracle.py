@@ -297,6 +297,79 @@ def test_pso_init_and_invariants(orc):
             prev = pbv
 
 
+def _pso_numpy(orc, prng, N, D, lo, hi, T, seed, f, w=0.5, c1=1.2, c2=1.5, vfrac=0.1,
+               swap_r=False):
+    """The reference PSO rule restated in numpy from include/pso.cuh: init
+    per coordinate a position U(lo, hi] then a velocity U(-vr, vr], vr =
+    0.1 (hi - lo) (pso.cuh:39-44); iteration t draws r1 then r2 per
+    coordinate (pso.cuh:105-106), v' = w v + c1 r1 (pb - x) + c2 r2 (gb - x),
+    x' = x + v' with w = 0.5, c1 = 1.2, c2 = 1.5 (pso.cuh:113-118, 302);
+    pBest on strict improvement; gBest = the lowest-index minimum of pBest
+    when it is strictly below the old gBest.  The uniforms come from the
+    generator streams alone (orc.stream_uniforms)."""
+    K = 2 * (T + 1)
+    U = np.zeros((N, D, K))
+    for i in range(N):
+        if prng == orc.PRNG_XORWOW:  # one stream per (start, coordinate)
+            for d in range(D):
+                U[i, d] = orc.stream_uniforms(prng, seed, 0, i, K, coord=d)
+        else:  # one stream per start; coordinate d of iteration t at 2Dt + 2d
+            s = orc.stream_uniforms(prng, seed, 0, i, K * D)
+            for t in range(T + 1):
+                for d in range(D):
+                    U[i, d, 2 * t:2 * t + 2] = s[2 * D * t + 2 * d:2 * D * t + 2 * d + 2]
+    vr = (hi - lo) * vfrac
+    X = lo + (hi + (-lo)) * U[:, :, 0]
+    V = -vr + (vr + vr) * U[:, :, 1]
+    pbx, pbv = X.copy(), np.array([f(x) for x in X])
+    gbv, gbx = np.inf, np.zeros(D)
+    m = int(np.argmin(pbv))
+    if pbv[m] < gbv:
+        gbv, gbx = pbv[m], pbx[m].copy()
+    for t in range(1, T + 1):
+        r1, r2 = U[:, :, 2 * t], U[:, :, 2 * t + 1]
+        if swap_r:
+            r1, r2 = r2, r1
+        V = w * V + c1 * r1 * (pbx - X) + c2 * r2 * (gbx[None, :] - X)
+        X = X + V
+        fx = np.array([f(x) for x in X])
+        better = fx < pbv
+        pbv = np.where(better, fx, pbv)
+        pbx = np.where(better[:, None], X, pbx)
+        m = int(np.argmin(pbv))
+        if pbv[m] < gbv:
+            gbv, gbx = pbv[m], pbx[m].copy()
+    return dict(x=X, v=V, pbest_x=pbx, pbest_val=pbv, gbest_x=gbx, gbest_val=gbv)
+
+
+@pytest.mark.parametrize("prng", ["xorwow", "philox"])
+def test_pso_rule_restated_from_streams(orc, prng):
+    """The oracle's PSO equals the reference rule written out in numpy from
+    the generator streams, bit for bit, on init and two iterations (N = 5,
+    D = 3, an asymmetric box so that a velocity range of 0.1 hi or 0.1 (hi +
+    lo) differs from 0.1 (hi - lo)).  A swap of c1 and c2, of r1 and r2, or
+    another velocity range gives another swarm, so each of those mistakes in
+    the oracle would fail this test."""
+    g = orc.PRNG_XORWOW if prng == "xorwow" else orc.PRNG_PHILOX
+    N, D, lo, hi, T, seed = 5, 3, -3.0, 7.0, 2, 42
+    obj = orc.Obj(orc.ROSENBROCK, D)
+    got = orc.pso_swarm(orc.ROSENBROCK, D, lo, hi, N, T, seed, prng=g)
+    want = _pso_numpy(orc, g, N, D, lo, hi, T, seed, obj.value)
+    for k in ("x", "v", "pbest_x", "pbest_val", "gbest_x"):
+        assert np.array_equal(got[k], want[k]), k
+    assert got["gbest_val"] == want["gbest_val"]
+    # T = 0: the initial velocities lie in (-0.1 R, 0.1 R]
+    init = orc.pso_swarm(orc.ROSENBROCK, D, lo, hi, N, 0, seed, prng=g)
+    R = hi - lo
+    assert np.all(init["v"] > -0.1 * R) and np.all(init["v"] <= 0.1 * R)
+    assert np.all(init["x"] > lo) and np.all(init["x"] <= hi)
+    assert np.array_equal(init["x"], init["pbest_x"])
+    # the comparison above has the power to see each plausible mistake
+    for bad in (dict(c1=1.5, c2=1.2), dict(swap_r=True), dict(vfrac=0.1 * hi / R)):
+        alt = _pso_numpy(orc, g, N, D, lo, hi, T, seed, obj.value, **bad)
+        assert not np.array_equal(alt["x"], got["x"]), bad
+
+
 # --- whole runs -----------------------------------------------------------------------
 def test_bfgs_reference_launch_kats(orc):
     # test/bfgs_tests.cu:266-329 -- Rastrigin from its minimum
