@@ -162,3 +162,25 @@ def test_graph_replays_match_eager():
                 assert np.array_equal(a.per_start[key], b.per_start[key]), (k, key)
             assert a.idx == b.idx and a.c.claimed == b.c.claimed
         z.run_batch(probs[1:], per_start=False)  # eager work between replays
+
+
+def test_graph_replay_follows_a_new_matrix():
+    """A replayed graph prepares the matrix of the call that replays it: a
+    second Gaussian matrix of the same shape (another device buffer) gives
+    the eager results of that matrix bit for bit, in the tile form and the
+    per-workgroup form, and the first matrix's results again after it."""
+    C1 = gauss_matrix(32)
+    C2 = ill_conditioned(32, 50.0, 11)
+    for form, threads in (("tile", 0), ("group", 256)):
+        def probs(C):
+            return [z.Problem(z.Gaussian(C), -5, 5, 48, 10000, 3, 49, 1e-8, 7)]
+        kw = dict(per_start=True, form=form, group_threads=threads)
+        e1 = z.run_batch(probs(C1), **kw)[0]
+        e2 = z.run_batch(probs(C2), **kw)[0]
+        assert not np.array_equal(e1.per_start["x"], e2.per_start["x"])
+        for k in range(3):  # capture with C1, replay with C2, replay with C1
+            C = (C1, C2, C1)[k]
+            want = (e1, e2, e1)[k]
+            g = z.run_batch(probs(C), use_graph=True, **kw)[0]
+            for key in ("status", "iter", "fval", "x"):
+                assert np.array_equal(want.per_start[key], g.per_start[key]), (form, k, key)

@@ -58,9 +58,13 @@ traj_fill_kernel(TrajOut t, int D)
 // major, and the same matrix in MFMA A-operand order: fragment (mb, kb) is
 // 64 consecutive doubles, lane l holding Cs[16 mb + l % 16][4 kb + l / 16]
 // (zero outside D x D), so a fragment load is one coalesced 512-byte read.
+// C is read through the problem descriptor in device memory (uploaded by
+// every call), never taken as a kernel argument, so that a replayed graph
+// prepares the matrix of the call that replays it.
 static __global__ void
-symmetrize_kernel(const double* C, double* Cs, double* Cf, int D)
+symmetrize_kernel(const ProblemDesc* desc, int k, double* Cs, double* Cf, int D)
 {
+  const double* C = desc[k].params;
   // element indices stay below D * D + (fragments) * 64 < 2^31: 32-bit
   // division (64-bit division is a long software sequence)
   const int e = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);

@@ -457,7 +457,7 @@ launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st, int t
     const int64_t na = dev::gaussian_aux_doubles(D);
     for (int k = 0; k < np; ++k) {
       double* cs = c.aux + k * na;
-      dev::symmetrize_kernel<<<int((na + 255) / 256), 256, 0, st>>>(c.desc[k].params, cs,
+      dev::symmetrize_kernel<<<int((na + 255) / 256), 256, 0, st>>>(c.d_desc, k, cs,
                                                                     cs + int64_t(D) * D, D);
     }
     if ((e = hipGetLastError()) != hipSuccess)
@@ -907,6 +907,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       key.push_back(uint64_t(c.probs.size()));
       key.push_back(uint64_t(c.cfg.form) << 32 | uint32_t(c.cfg.threads));
       key.push_back(uint64_t(c.xorwow));
+      key.push_back(uint64_t(c.metrics));
     }
     key.push_back(uint64_t(concurrent));
     hipGraphExec_t exec = nullptr;
