@@ -96,6 +96,13 @@ static_assert(zeus::ZeusObjective<UserRosenbrock<2>, 2>);
 static_assert(zeus::FnTraits<UserGaussian<4>>::arity == 4);
 static_assert(zeus::HasAdjoint<UserSphereAdjoint> && !zeus::HasAdjoint<UserRosenbrock<2>>);
 
+// DeviceMatrix contract (test/device_matrix_tests.cu): owned device-heap
+// storage that cannot be copied, assigned or moved.
+static_assert(!std::is_copy_constructible_v<DeviceMatrix<int>>);
+static_assert(!std::is_copy_assignable_v<DeviceMatrix<int>>);
+static_assert(!std::is_move_constructible_v<DeviceMatrix<double>>);
+static_assert(std::is_destructible_v<DeviceMatrix<int>>);
+
 static int failures = 0;
 #define CHECK(cond)                                                                    \
   do {                                                                                 \
@@ -163,9 +170,65 @@ host_only()
   return failures == 0 ? 0 : 1;
 }
 
+// One thread builds a rows x cols DeviceMatrix on the device heap, fills
+// element (i, j) with base + (i cols + j), copies it out row-major and
+// reports its shape; the matrix is freed when it leaves scope.
+template <class T>
+__global__ void
+devmat_fill_kernel(T* out, int rows, int cols, T base, int* shape)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0)
+    return;
+  DeviceMatrix<T> M(rows, cols);
+  if (!M.valid()) {
+    shape[0] = shape[1] = -1;
+    return;
+  }
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < cols; ++j)
+      M(i, j) = static_cast<T>(base + static_cast<T>(i * cols + j));
+  M.set(0, 0, M(0, 0)); // set() writes through the same storage
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < cols; ++j)
+      out[i * cols + j] = M(i, j);
+  shape[0] = M.rows();
+  shape[1] = M.cols();
+}
+
+template <class T>
+bool
+devmat_fill_and_check(int rows, int cols, T base)
+{
+  const int n = rows * cols;
+  void *d_out = nullptr, *d_shape = nullptr;
+  if (zeus_alloc(&d_out, sizeof(T) * n) != ZEUS_OK || zeus_alloc(&d_shape, 2 * sizeof(int)) != ZEUS_OK)
+    return false;
+  devmat_fill_kernel<T><<<1, 1>>>(static_cast<T*>(d_out), rows, cols, base,
+                                  static_cast<int*>(d_shape));
+  bool ok = hipGetLastError() == hipSuccess && zeus_synchronize() == ZEUS_OK;
+  std::vector<T> h(n);
+  int shape[2] = {0, 0};
+  ok = ok && zeus_memcpy_d2h(h.data(), d_out, sizeof(T) * n) == ZEUS_OK &&
+       zeus_memcpy_d2h(shape, d_shape, sizeof(shape)) == ZEUS_OK;
+  (void)zeus_free(d_out);
+  (void)zeus_free(d_shape);
+  ok = ok && shape[0] == rows && shape[1] == cols;
+  for (int i = 0; ok && i < n; ++i)
+    ok = h[i] == static_cast<T>(base + static_cast<T>(i));
+  return ok;
+}
+
 int
 gpu_cases()
 {
+  // DeviceMatrix: fill and read back (test/device_matrix_tests.cu), int and
+  // double, small and rectangular, repeated construction and destruction
+  CHECK(devmat_fill_and_check<int>(3, 4, 10));
+  CHECK(devmat_fill_and_check<double>(5, 2, 1.5));
+  for (int k = 0; k < 5; ++k)
+    CHECK(devmat_fill_and_check<int32_t>(2, 3, k * 100));
+  std::printf("{\"case\": \"device_matrix\", \"ok\": %d}\n", failures == 0);
+
   // buffers: deep copy, move, copyToHost (cuda_buffer_tests.cu)
   {
     std::array<double, 4> h{1, 2, 3, 4};

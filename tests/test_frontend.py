@@ -153,3 +153,11 @@ def test_frontend_trajectory_file(orc, gpu_cases):
                          (6, t["coords"][it, 0, i]), (7, t["coords"][it, 1, i])):
             assert (r[col] == "NaN") if np.isnan(arr) else float(r[col]) == arr
         assert int(r[4]) == t["status"][it, i] and float(r[5]) == 0.0
+
+
+@pytest.mark.gpu
+def test_frontend_device_matrix(gpu_cases):
+    """DeviceMatrix (test/device_matrix_tests.cu): a device-heap matrix built,
+    filled and read back by one device thread, int and double, repeatedly;
+    its copy and move operations are deleted (static_asserts)."""
+    assert gpu_cases["device_matrix"]["ok"] == 1

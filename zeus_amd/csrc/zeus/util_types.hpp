@@ -60,6 +60,8 @@ public:
   }
   DeviceMatrix(const DeviceMatrix&) = delete;
   DeviceMatrix& operator=(const DeviceMatrix&) = delete;
+  DeviceMatrix(DeviceMatrix&&) = delete; // the storage is owned by one object
+  DeviceMatrix& operator=(DeviceMatrix&&) = delete;
   __device__ ~DeviceMatrix() { release(); }
   __device__ void release()
   {
