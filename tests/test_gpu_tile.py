@@ -1,6 +1,9 @@
-"""GPU parity of the tile form (16 starts per workgroup, Gaussian objective
+"""GPU parity of the tile forms (16 starts per workgroup, Gaussian objective
 evaluated for the tile on FP64 MFMA, H kept as B + rank-2 pairs) against the
-CPU oracle, through the C ABI.
+CPU oracle, through the C ABI: the single form (one trial per start per
+round) and the paired form (two half-tiles out of phase, two trials per
+start per round), which must agree with each other bit for bit when no
+early stop intervenes.
 
 The tile form sums in another order than the serial oracle, so parity is to
 tolerance: equal statuses, |df| <= 1e-8 max(1, |f|), ||dx||_inf <= 1e-6.
@@ -41,12 +44,16 @@ def check_against_oracle(ps, o, *, x_tol=X_TOL):
     assert np.abs(ps["x"] - o.x).max() <= x_tol
 
 
-def test_tile_gaussian150_matches_oracle(orc):
+FORMS = ["tile", "tile_paired"]
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_tile_gaussian150_matches_oracle(orc, form):
     """C2 problem, 64 starts (4 tiles), no early stop."""
     C = gauss_matrix(150)
     N = 64
     r = z.Zeus(z.Gaussian(C), -5, 5, N, 10000, 10, N + 1, "gaussian", 1e-8, 42, 0,
-               form="tile", per_start=True)
+               form=form, per_start=True)
     o = orc.run(orc.GAUSSIAN, 150, -5, 5, N, 10000, 10, N + 1, 1e-8, 42, params=C.reshape(-1))
     ps = r.per_start
     assert set(o.status) == {1}
@@ -60,14 +67,15 @@ def test_tile_gaussian150_matches_oracle(orc):
     assert r.sum_grads == int(ps["grads"].sum()) and r.sum_iter == int(ps["iter"].sum())
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("D,kappa,N", [(32, 1e3, 40), (150, 1e3, 24)])
-def test_tile_folds_pair_list(orc, D, kappa, N):
+def test_tile_folds_pair_list(orc, D, kappa, N, form):
     """Ill-conditioned quadratics take more than 32 iterations, so the rank-2
     pair list is folded into the explicit B at least once (4 times at
     D = 150); the minimizer and statuses still match the oracle."""
     C = ill_conditioned(D, kappa, 5)
     r = z.Zeus(z.Gaussian(C), -5, 5, N, 10000, 5, N + 1, "gaussian", 1e-8, 42, 0,
-               form="tile", per_start=True)
+               form=form, per_start=True)
     o = orc.run(orc.GAUSSIAN, D, -5, 5, N, 10000, 5, N + 1, 1e-8, 42, params=C.reshape(-1))
     ps = r.per_start
     assert set(o.status) == {1} and o.iter.min() > 32
@@ -77,7 +85,8 @@ def test_tile_folds_pair_list(orc, D, kappa, N):
     assert np.abs(np.median(ps["iter"]) - np.median(o.iter)) <= 3
 
 
-def test_tile_batch_of_problems(orc):
+@pytest.mark.parametrize("form", FORMS)
+def test_tile_batch_of_problems(orc, form):
     """Two Gaussian-32 problems with partial last tiles (20 and 37 starts) and
     a Gaussian-150 problem in one call: tiles map to their own problem."""
     C1 = ill_conditioned(32, 10.0, 1)
@@ -86,7 +95,7 @@ def test_tile_batch_of_problems(orc):
     probs = [z.Problem(z.Gaussian(C1), -5, 5, 20, 10000, 3, 21, 1e-8, 7),
              z.Problem(z.Gaussian(C2), -3, 3, 37, 10000, 4, 38, 1e-8, 8, stream=3),
              z.Problem(z.Gaussian(C3), -5, 5, 17, 10000, 2, 18, 1e-8, 9, stream=1)]
-    res = z.run_batch(probs, form="tile", per_start=True)
+    res = z.run_batch(probs, form=form, per_start=True)
     for pr, r, C in zip(probs, res, (C1, C2, C3)):
         D = C.shape[0]
         o = orc.run(orc.GAUSSIAN, D, pr.lower, pr.upper, pr.N, pr.MAX_ITER, pr.PSO_ITER,
@@ -96,13 +105,14 @@ def test_tile_batch_of_problems(orc):
         assert r.idx == o.best["idx"] or close_f(r.fval, o.best["fval"])
 
 
-def test_tile_surrender_and_random_starts(orc):
+@pytest.mark.parametrize("form", FORMS)
+def test_tile_surrender_and_random_starts(orc, form):
     """MAX_ITER = 3 ends every start with status 0 after 3 iterations; with
     PSO_ITER < 0 the starts are drawn in the BFGS from the Philox stream."""
     C = gauss_matrix(150)
     N = 32
     r = z.Zeus(z.Gaussian(C), -5, 5, N, 3, -1, N + 1, "gaussian", 1e-8, 11, 0,
-               form="tile", per_start=True)
+               form=form, per_start=True)
     o = orc.run(orc.GAUSSIAN, 150, -5, 5, N, 3, -1, N + 1, 1e-8, 11, params=C.reshape(-1))
     ps = r.per_start
     assert set(ps["status"]) == {0} and set(ps["iter"]) == {3}
@@ -110,22 +120,24 @@ def test_tile_surrender_and_random_starts(orc):
     assert r.c.surrendered == N
 
 
-def test_tile_max_iter_zero(orc):
+@pytest.mark.parametrize("form", FORMS)
+def test_tile_max_iter_zero(orc, form):
     C = gauss_matrix(32)
-    r = z.Zeus(z.Gaussian(C), -5, 5, 16, 0, 2, 17, "gaussian", 1e-8, 3, 0, form="tile",
+    r = z.Zeus(z.Gaussian(C), -5, 5, 16, 0, 2, 17, "gaussian", 1e-8, 3, 0, form=form,
                per_start=True)
     o = orc.run(orc.GAUSSIAN, 32, -5, 5, 16, 0, 2, 17, 1e-8, 3, params=C.reshape(-1))
     check_against_oracle(r.per_start, o)
     assert set(r.per_start["iter"]) == {0}
 
 
-def test_tile_early_stop(orc):
+@pytest.mark.parametrize("form", FORMS)
+def test_tile_early_stop(orc, form):
     """required = 5 of 160 starts: the call stops once 5 have converged, every
     start is converged or stopped, and stopped starts stopped no later than
     the oracle's unstopped run converges them."""
     C = gauss_matrix(150)
     N = 160
-    r = z.Zeus(z.Gaussian(C), -5, 5, N, 10000, 10, 5, "gaussian", 1e-8, 42, 0, form="tile",
+    r = z.Zeus(z.Gaussian(C), -5, 5, N, 10000, 10, 5, "gaussian", 1e-8, 42, 0, form=form,
                per_start=True)
     ps = r.per_start
     assert r.c.claimed >= 5 and r.c.claimed + r.c.stopped == N
@@ -144,7 +156,7 @@ def test_tile_is_tuned_for_the_headline():
     r = z.Zeus(z.Gaussian(C), -5, 5, 512, 10000, 10, 513, "gaussian", 1e-8, 42, 0)
     assert r.status == 1 and r.c.claimed == 512
     form, threads = z.tuned_config(4, 150, 0)
-    assert form in (2, 3) and threads in (256, 512, 1024)
+    assert form in (2, 3, 5) and threads in (256, 512, 1024)
 
 
 def test_graph_replays_match_eager():
@@ -171,7 +183,7 @@ def test_graph_replay_follows_a_new_matrix():
     per-workgroup form, and the first matrix's results again after it."""
     C1 = gauss_matrix(32)
     C2 = ill_conditioned(32, 50.0, 11)
-    for form, threads in (("tile", 0), ("group", 256)):
+    for form, threads in (("tile", 0), ("tile_paired", 0), ("group", 256)):
         def probs(C):
             return [z.Problem(z.Gaussian(C), -5, 5, 48, 10000, 3, 49, 1e-8, 7)]
         kw = dict(per_start=True, form=form, group_threads=threads)
@@ -184,3 +196,21 @@ def test_graph_replay_follows_a_new_matrix():
             g = z.run_batch(probs(C), use_graph=True, **kw)[0]
             for key in ("status", "iter", "fval", "x"):
                 assert np.array_equal(want.per_start[key], g.per_start[key]), (form, k, key)
+
+
+@pytest.mark.parametrize("prng", [z.PRNGType.PHILOX, z.PRNGType.XORWOW], ids=["philox", "xorwow"])
+@pytest.mark.parametrize("N", [512, 333])
+def test_paired_equals_single_without_early_stop(prng, N):
+    """The paired form evaluates the same points with the same product units
+    and takes the same decisions as the single form, only two trials of a
+    start per round: without an early stop every per-start output is the
+    same, bit for bit (also with a ragged last tile)."""
+    C = gauss_matrix(150)
+    f = z.Gaussian(C)
+    a = z.Zeus(f, -5, 5, N, 10000, 10, N + 1, "gaussian", 1e-8, 42, 0, prng=prng, form="tile",
+               per_start=True)
+    b = z.Zeus(f, -5, 5, N, 10000, 10, N + 1, "gaussian", 1e-8, 42, 0, prng=prng,
+               form="tile_paired", per_start=True)
+    for key in ("status", "iter", "ls_evals", "grads", "fval", "gnorm", "x"):
+        assert np.array_equal(a.per_start[key], b.per_start[key]), key
+    assert a.idx == b.idx and a.fval == b.fval and a.sum_ls_evals == b.sum_ls_evals

@@ -230,30 +230,37 @@ struct Ops {
     return 0;
   }
 
-  static hipError_t bfgs_tile(const dev::TileArgs& a, int grid, hipStream_t s)
+  // paired = 0: single tile form (bfgs_tile.hpp); 1: paired (bfgs_tile_paired.hpp)
+  static hipError_t bfgs_tile(const dev::TileArgs& a, int grid, int paired, hipStream_t s)
   {
     if constexpr (!kTileOk) {
       return hipErrorInvalidConfiguration;
     } else {
       if (fault_take_launch())
         return hipErrorLaunchFailure;
-      dev::bfgs_tile_kernel<D><<<grid, dev::kTileThreads, 0, s>>>(a);
+      if (paired)
+        dev::bfgs_tile_paired_kernel<D><<<grid, dev::kTileThreads, 0, s>>>(a);
+      else
+        dev::bfgs_tile_kernel<D><<<grid, dev::kTileThreads, 0, s>>>(a);
       return hipGetLastError();
     }
   }
-  static int tile_blocks_per_cu()
+  static int tile_blocks_per_cu(int paired)
   {
     if constexpr (!kTileOk) {
       return 1;
     } else {
-      static int cached = 0;
-      if (cached == 0) {
+      static int cached[2] = {0, 0};
+      int& c = cached[paired ? 1 : 0];
+      if (c == 0) {
         int n = 0;
-        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &n, reinterpret_cast<const void*>(&dev::bfgs_tile_kernel<D>), dev::kTileThreads, 0);
-        cached = (e == hipSuccess && n > 0) ? n : 1;
+        const void* k = paired ? reinterpret_cast<const void*>(&dev::bfgs_tile_paired_kernel<D>)
+                               : reinterpret_cast<const void*>(&dev::bfgs_tile_kernel<D>);
+        const hipError_t e =
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, dev::kTileThreads, 0);
+        c = (e == hipSuccess && n > 0) ? n : 1;
       }
-      return cached;
+      return c;
     }
   }
 
