@@ -44,9 +44,9 @@ def test_pso_phases_equal_one_call(orc, case):
 @pytest.mark.parametrize("g", GENS, ids=[x.name for x in GENS])
 def test_pso_positions_and_velocities_equal_oracle(orc, g):
     """Positions and velocities after init and after two steps equal the
-    oracle's swarm bit for bit (lane form, Rosenbrock-3 on [-3, 7]); the
+    oracle's swarm bit for bit (lane form, Rosenbrock-4 on [-3, 7]); the
     oracle's PSO is pinned to the reference rule by a numpy restatement."""
-    N, D, lo, hi = 37, 3, -3.0, 7.0
+    N, D, lo, hi = 37, 4, -3.0, 7.0
     sw = z.Swarm(z.Rosenbrock(D), lo, hi, N, 42, prng=g).init()
     for T in (0, 1, 2):
         if T:

@@ -35,31 +35,51 @@ import numpy as np
 import zeus_amd as z
 
 
-def _mixed_problems():
+def mixed_specs():
+    """The 64 problems of the mixed batch as plain records: kind (objective
+    name), dim, box half-width, starts, required, seed and stream.
+    Kinds and start counts come from a numpy generator seeded with 42."""
     rng = np.random.default_rng(42)
-    probs, names = [], []
+    specs = []
     for i in range(64):
         k = int(rng.integers(0, 7))
         n = int(round(math.exp(rng.uniform(math.log(128), math.log(4096)))))
-        req = min(100, n)
         if k <= 2:
-            dim = (2, 10, 50)[k]
-            probs.append(z.Problem(z.Rosenbrock(dim), -5, 5, n, 10000, 10, req, 1e-8, 42 + i,
-                                   stream=i))
-            names.append(f"rosenbrock{dim}")
+            kind, dim, box = "rosenbrock", (2, 10, 50)[k], 5.0
         elif k == 3:
-            probs.append(z.Problem(z.Rastrigin(20), -5.12, 5.12, n, 10000, 10, req, 1e-8,
-                                   42 + i, stream=i))
-            names.append("rastrigin20")
+            kind, dim, box = "rastrigin", 20, 5.12
         elif k == 4:
-            probs.append(z.Problem(z.Ackley(10), -32.768, 32.768, n, 10000, 10, req, 1e-8,
-                                   42 + i, stream=i))
-            names.append("ackley10")
+            kind, dim, box = "ackley", 10, 32.768
         else:
-            dim = (32, 150)[k - 5]
-            probs.append(z.Problem(z.Gaussian(z.make_spd(42, i, dim)), -5, 5, n, 10000, 10, req,
-                                   1e-8, 42 + i, stream=i))
-            names.append(f"gaussian{dim}")
+            kind, dim, box = "gaussian", (32, 150)[k - 5], 5.0
+        specs.append(dict(kind=kind, dim=dim, box=box, n=n, required=min(100, n), seed=42 + i,
+                          stream=i, name=f"{kind}{dim}"))
+    return specs
+
+
+def mixed_objective(spec):
+    """The objective of one mixed-batch record (Gaussian: C = A A^T / D + I,
+    A ~ N(0, 1) drawn from (42, stream))."""
+    kind, dim = spec["kind"], spec["dim"]
+    if kind == "rosenbrock":
+        return z.Rosenbrock(dim)
+    if kind == "rastrigin":
+        return z.Rastrigin(dim)
+    if kind == "ackley":
+        return z.Ackley(dim)
+    return z.Gaussian(z.make_spd(42, spec["stream"], dim))
+
+
+def _mixed_problems(max_starts=None, max_iter=10000, early_stop=True):
+    """The mixed batch as Problems; max_starts caps every start count and
+    early_stop=False asks for every start (required = N + 1)."""
+    probs, names = [], []
+    for sp in mixed_specs():
+        n = sp["n"] if max_starts is None else min(sp["n"], max_starts)
+        req = min(sp["required"], n) if early_stop else n + 1
+        probs.append(z.Problem(mixed_objective(sp), -sp["box"], sp["box"], n, max_iter, 10, req,
+                               1e-8, sp["seed"], stream=sp["stream"]))
+        names.append(sp["name"])
     return probs, names
 
 
