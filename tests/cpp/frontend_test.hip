@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -273,6 +274,31 @@ gpu_cases()
     UserGaussian<D> g(c);
     auto r = zeus::Zeus(g, -5.0, 5.0, 64, 10000, 10, 65, "gaussian", 1e-8, 42, 0);
     print_result("gaussian20_group_dual", r);
+  }
+  // built-in objectives (zeus/builtins.hpp): the library's kernels, autotuned
+  {
+    constexpr std::size_t D = 150;
+    auto C = std::make_unique<std::array<std::array<double, D>, D>>();
+    for (std::size_t i = 0; i < D; ++i)
+      for (std::size_t j = 0; j < D; ++j)
+        (*C)[i][j] = i == j ? 1.0 + (D - 1) * 0.5 : 0.5;
+    const zeus::Gaussian<D> g(*C);
+    auto r = zeus::Zeus(g, -5.0, 5.0, 64, 10000, 10, 65, "gaussian", 1e-8, 42, 0);
+    print_result("gaussian150_builtin", r);
+    int form = 0, threads = 0;
+    CHECK(zeus_tuned_config(ZEUS_OBJ_GAUSSIAN, int(D), ZEUS_GRAD_ADJOINT, &form, &threads) ==
+          ZEUS_OK);
+    std::printf("{\"case\": \"gaussian150_builtin_form\", \"form\": %d, \"threads\": %d}\n",
+                form, threads);
+    // the headline shape: 4096 starts, 100 required, Philox draws
+    auto h = zeus::Zeus(g, -5.0, 5.0, 4096, 10000, 10, 100, "gaussian", 1e-8, 42, 0, true,
+                        zeus::PRNGType::PHILOX);
+    print_result("gaussian150_builtin_headline", h);
+  }
+  {
+    auto r = zeus::Zeus(zeus::obj::Rosenbrock<10>{}, -5.0, 5.0, 256, 10000, 10, 257,
+                        "rosenbrock", 1e-8, 42, 0);
+    print_result("rosenbrock10_builtin", r);
   }
   // adjoint picked up from value_and_gradient
   {

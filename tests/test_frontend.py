@@ -108,6 +108,42 @@ def test_frontend_gaussian_matrix_member(orc, gpu_cases):
 
 
 @pytest.mark.gpu
+def test_frontend_builtin_gaussian150(orc, gpu_cases):
+    """zeus::Zeus(zeus::Gaussian<150>{C}, ...) -- the reference's example
+    Gaussian as a built-in -- runs the library's kernels (the tuned form of
+    the class is a tile form) and agrees with the oracle: 64 starts without
+    early stop converge to the minimum; the headline shape (4096 starts, 100
+    required, Philox) reaches its required count with the minimum."""
+    D = 150
+    C = np.full((D, D), 0.5)
+    np.fill_diagonal(C, 1.0 + (D - 1) * 0.5)
+    assert gpu_cases["gaussian150_builtin_form"]["form"] in (3, 5)  # ZEUS_FORM_TILE{,_PAIRED}
+    got = gpu_cases["gaussian150_builtin"]
+    want = orc.run(orc.GAUSSIAN, D, -5, 5, 64, 10000, 10, 65, 1e-8, 42, params=C.reshape(-1),
+                   early_stop=False)
+    assert got["status"] == want.best["status"] == 1
+    assert got["claimed"] == _counts(want)["claimed"] == 64
+    assert abs(got["fval"] - want.best["fval"]) <= 1e-8
+    assert np.abs(np.array(got["x"]) - want.best_x).max() <= 1e-6
+    h = gpu_cases["gaussian150_builtin_headline"]
+    assert h["status"] == 1 and h["claimed"] >= 100 and abs(h["fval"]) <= 1e-8
+    assert h["claimed"] + h["stopped"] == 4096
+    assert np.abs(np.array(h["x"])).max() <= 1e-6
+
+
+@pytest.mark.gpu
+def test_frontend_builtin_rosenbrock10(orc, gpu_cases):
+    """zeus::obj::Rosenbrock<10> is a built-in: the library's autotuned
+    kernels give the oracle's best start (status, value, minimizer)."""
+    got = gpu_cases["rosenbrock10_builtin"]
+    want = orc.run(orc.ROSENBROCK, 10, -5, 5, 256, 10000, 10, 257, 1e-8, 42, early_stop=False)
+    assert got["status"] == want.best["status"] == 1
+    assert abs(got["fval"] - want.best["fval"]) <= 1e-8
+    assert np.abs(np.array(got["x"]) - want.best_x).max() <= 1e-6
+    assert abs(got["claimed"] - _counts(want)["claimed"]) <= 256 // 50
+
+
+@pytest.mark.gpu
 def test_frontend_adjoint_functor_bitwise(orc, gpu_cases):
     """A functor with value_and_gradient runs the adjoint path; its arithmetic
     is the oracle's sphere adjoint, so the per-thread result is bit-exact."""

@@ -70,6 +70,10 @@ template <std::size_t D>
 struct Rosenbrock {
   static constexpr std::size_t arity = D;
   static constexpr int id = kRosenbrock;
+  // run by zeus::Zeus on the library's built-in kernels (no parameters)
+  static constexpr int zeus_objective = kRosenbrock;
+  const double* zeus_params() const { return nullptr; }
+  long long zeus_n_params() const { return 0; }
 
   template <class T, class X>
   ZEUS_HD T eval(const X& x) const
@@ -113,6 +117,10 @@ template <std::size_t D>
 struct Rastrigin {
   static constexpr std::size_t arity = D;
   static constexpr int id = kRastrigin;
+  // run by zeus::Zeus on the library's built-in kernels (no parameters)
+  static constexpr int zeus_objective = kRastrigin;
+  const double* zeus_params() const { return nullptr; }
+  long long zeus_n_params() const { return 0; }
 
   template <class T, class X>
   ZEUS_HD T eval(const X& x) const
@@ -149,6 +157,10 @@ template <std::size_t D>
 struct Ackley {
   static constexpr std::size_t arity = D;
   static constexpr int id = kAckley;
+  // run by zeus::Zeus on the library's built-in kernels (no parameters)
+  static constexpr int zeus_objective = kAckley;
+  const double* zeus_params() const { return nullptr; }
+  long long zeus_n_params() const { return 0; }
 
   template <class T, class X>
   ZEUS_HD T eval(const X& x) const
@@ -443,6 +455,10 @@ template <std::size_t D>
 struct Sphere {
   static constexpr std::size_t arity = D;
   static constexpr int id = kSphere;
+  // run by zeus::Zeus on the library's built-in kernels (no parameters)
+  static constexpr int zeus_objective = kSphere;
+  const double* zeus_params() const { return nullptr; }
+  long long zeus_n_params() const { return 0; }
   template <class T, class X>
   ZEUS_HD T eval(const X& x) const
   {

@@ -25,6 +25,7 @@
 #include "batch.hpp"
 #include "bfgs_group.hpp"
 #include "bfgs_thread.hpp"
+#include "builtins.hpp"
 #include "config.hpp"
 #include "dual.hpp"
 #include "exception.hpp"
@@ -354,6 +355,69 @@ run(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO
   return res;
 }
 
+// A built-in objective (builtins.hpp) through zeus_run_batch: the library's
+// kernels for it, autotuned.  Returns false when the library does not
+// instantiate the objective at this dimension (the caller then runs the
+// functor path).
+template <class F, int D>
+bool
+run_builtin(const F& f, double lower, double upper, std::size_t N, int MAX_ITER, int PSO_ITER,
+            int required, std::string_view fun_name, double tol, int seed, int run_id, int prng,
+            Gradient gradient, Result<D>& res)
+{
+  const std::string name(fun_name);
+  zeus_problem p{};
+  p.objective = F::zeus_objective;
+  p.dim = D;
+  p.lower = lower;
+  p.upper = upper;
+  p.n_starts = int64_t(N);
+  p.max_iter = MAX_ITER;
+  p.pso_iter = PSO_ITER;
+  p.required = required;
+  p.seed = uint32_t(seed);
+  p.stream = 0;
+  p.prng = prng;
+  p.tol = tol;
+  p.params = f.zeus_params();
+  p.n_params = int64_t(f.zeus_n_params());
+  p.name = name.c_str();
+  zeus_options o{};
+  o.gradient = gradient == Gradient::Dual ? ZEUS_GRAD_DUAL : ZEUS_GRAD_ADJOINT;
+  o.form = ZEUS_FORM_GROUP;
+  zeus_result r{};
+  double bx[D];
+  const int rc = zeus_run_batch(&p, 1, &o, &r, bx, nullptr, nullptr);
+  if (rc == ZEUS_E_UNSUPPORTED)
+    return false;
+  res = Result<D>{};
+  if (rc == ZEUS_E_ALLOC || rc == ZEUS_E_LAUNCH || rc == ZEUS_E_NODEVICE) {
+    res.status = rc == ZEUS_E_ALLOC ? kStatusMalloc : kStatusKernel; // zeus.cuh:108-116
+    return true;
+  }
+  if (rc != ZEUS_OK)
+    throw std::invalid_argument(std::string("zeus::Zeus: ") + zeus_status_string(rc));
+  res.idx = int(r.idx);
+  res.status = r.status;
+  res.iter = r.iter;
+  res.fval = r.fval;
+  res.gradientNorm = r.gradient_norm;
+  for (int d = 0; d < D; ++d)
+    res.coordinates[d] = bx[d];
+  res.c.claimed = r.claimed;
+  res.c.actual = r.actual;
+  res.c.stopped = r.stopped;
+  res.c.surrendered = r.surrendered;
+  res.ms_opt = r.ms_opt;
+  if (results_table_enabled()) { // zeus.cuh:172-183
+    const double err = util::calculateEuclideanError(name, res.coordinates, D);
+    util::appendResultsToTsv<std::size_t(D)>(D, int(N), name, 0.f, float(r.ms_pso),
+                                             float(r.ms_opt), 0.f, MAX_ITER, PSO_ITER, err,
+                                             run_id, res);
+  }
+  return true;
+}
+
 } // namespace detail
 
 template <typename Function, std::size_t ZEUS_DIM = FnTraits<Function>::arity>
@@ -382,6 +446,16 @@ Zeus(Function const& f, double lower, double upper, std::size_t N, int MAX_ITER,
     spec.alpha = parallel;
     tr = &spec;
   }
+  // built-in objectives run on the library's kernels (parallel form, no
+  // trajectory), unless the library lacks this dimension
+  if constexpr (IsBuiltin<Function>)
+    if (parallel && tr == nullptr && N > 0 && MAX_ITER >= 0) {
+      Result<ZEUS_DIM> res;
+      if (detail::run_builtin<Function, D>(f, lower, upper, N, MAX_ITER, PSO_ITER,
+                                           requiredConverged, fun_name, tolerance, seed, run,
+                                           kind, gradient, res))
+        return res;
+    }
   if constexpr (HasAdjoint<Function>)
     if (gradient == Gradient::Adjoint)
       return detail::run<Function, D, Gradient::Adjoint>(f, lower, upper, N, MAX_ITER, PSO_ITER,
