@@ -4,21 +4,20 @@
 Workload (one step = one zeus::Zeus call, PSO -> BFGS -> argmin -> results to
 the host): f(x) = 0.5 x^T C x with C_ii = 1 + 149 * 0.5, C_ij = 0.5, box
 [-5, 5]^150, 4096 starts, 10 PSO iterations, MAX_ITER 10000, tol 1e-8,
-seed 42, 100 required convergences (early stop), FP64, adjoint gradient.
-C is uploaded once before the timed region (inputs resident in HBM).
+seed 42, 100 required convergences (early stop), FP64, adjoint gradient,
+every draw from the counter-based Philox4x32-10 stream.  C is uploaded once
+before the timed region (inputs resident in HBM).
 
-Also measured on the same GPU: the dual-number gradient on the same call
-(the t_dual / t_adjoint ratio), the BFGS kernel against the FP64 peak, and
-the CPU oracle on a bounded sample of the same workload.  The starts are
-drawn with the default generator (XORWOW, as in the reference); --prng picks
-another.  --throughput adds the no-early-stop rate (required = N + 1) and
---generators the rate under each generator; both launch the same BFGS
-kernel on other workloads, so they are off by default, where every launch
-of that kernel in the run is a headline launch (its profiled average
-duration is then the one the roofline uses).
+Also measured on the same GPU, after the timed headline calls (so that the
+first warmup + steps launches of the BFGS kernel in a run are the headline
+ones): the same call under the XORWOW generator (the reference's default),
+the no-early-stop throughput (required = N + 1), the dual-number gradient on
+the same call (the t_dual / t_adjoint ratio), and the CPU oracle on a
+bounded sample of the same workload in two modes (the reference-literal
+O(D^4) update and the rank-2 O(D^2) update, dual gradient both).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--prng G] [--throughput]
-                    [--generators]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--prng G] [--generators]
+                    [--no-cpu]
 """
 from __future__ import annotations
 
@@ -76,29 +75,49 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(threads: int):
-    """Oracle (reference-literal O(D^4) update, dual gradient) on a bounded
-    sample: PSO over all 4096 particles, then BFGS from the first K = 2T
-    pBest starts to convergence (no early stop).  converged/s is
-    K_converged / (t_BFGS + t_PSO * K / 4096)."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(threads: int, prng: int):
+    """Oracle on a bounded sample, OpenMP over `threads` host threads: PSO
+    over all 4096 particles, then BFGS from the first K pBest starts to
+    convergence (no early stop), dual gradient, in two modes: the
+    reference-literal O(D^4) update (K = 2T) and the rank-2 O(D^2) update
+    (K = 8T).  converged/s is K_converged / (t_BFGS + t_PSO * K / 4096);
+    `value` is the reference-literal mode."""
     from oracle import binding as orc
     C = gauss_matrix().reshape(-1)
     t0 = time.perf_counter()
-    pbx, _, _, _ = orc.pso(orc.GAUSSIAN, D, -5.0, 5.0, N, PSO_ITER, SEED, params=C)
+    pbx, _, _, _ = orc.pso(orc.GAUSSIAN, D, -5.0, 5.0, N, PSO_ITER, SEED, params=C, prng=prng)
     t_pso = time.perf_counter() - t0
-    K = 2 * threads
-    t0 = time.perf_counter()
-    r = orc.run(orc.GAUSSIAN, D, -5.0, 5.0, K, MAX_ITER, PSO_ITER, K + 1, TOL, SEED, params=C,
-                x0=pbx[:K], grad=orc.GRAD_DUAL, update_mode=orc.UPDATE_LITERAL,
-                threads=threads, early_stop=False)
-    t_bfgs = time.perf_counter() - t0
-    conv = int((r.status == 1).sum())
-    t_eff = t_bfgs + t_pso * K / N
-    return {"value": conv / t_eff, "unit": "converged starts/s", "cores": threads,
-            "kind": "port",
-            "sample": (f"oracle reference-literal (O(D^4) update, dual gradient), OpenMP "
-                       f"{threads} threads: PSO over {N} particles ({t_pso:.2f} s) then BFGS "
-                       f"from the first {K} pBest starts ({t_bfgs:.2f} s, {conv} converged)")}
+    modes = {}
+    for name, mode, K in (("literal", orc.UPDATE_LITERAL, 2 * threads),
+                          ("rank2", orc.UPDATE_RANK2, 8 * threads)):
+        t0 = time.perf_counter()
+        r = orc.run(orc.GAUSSIAN, D, -5.0, 5.0, K, MAX_ITER, PSO_ITER, K + 1, TOL, SEED,
+                    params=C, x0=pbx[:K], grad=orc.GRAD_DUAL, update_mode=mode,
+                    threads=threads, early_stop=False, prng=prng)
+        t_bfgs = time.perf_counter() - t0
+        conv = int((r.status == 1).sum())
+        modes[name] = {"value": conv / (t_bfgs + t_pso * K / N), "starts": K,
+                       "converged": conv, "s_bfgs": t_bfgs}
+    lit, r2 = modes["literal"], modes["rank2"]
+    return {"value": lit["value"], "unit": "converged starts/s", "cores": threads,
+            "kind": "port", "cpu_model": cpu_model(), "modes": modes,
+            "sample": (f"oracle, OpenMP {threads} threads ({cpu_model()}): PSO over {N} "
+                       f"particles ({t_pso:.2f} s), then BFGS with the dual gradient from the "
+                       f"first pBest starts: reference-literal O(D^4) update {lit['starts']} "
+                       f"starts ({lit['s_bfgs']:.2f} s, {lit['converged']} converged), rank-2 "
+                       f"O(D^2) update {r2['starts']} starts ({r2['s_bfgs']:.2f} s, "
+                       f"{r2['converged']} converged)")}
 
 
 def main() -> int:
@@ -108,12 +127,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--dual-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--throughput", action="store_true")
     ap.add_argument("--generators", action="store_true",
-                    help="also time the workload under each generator")
-    ap.add_argument("--prng", choices=("xorwow", "philox", "sobol"), default="xorwow",
-                    help="generator of the timed calls (default: the library's and the "
-                         "reference's default, XORWOW)")
+                    help="also time the workload under the SOBOL generator")
+    ap.add_argument("--prng", choices=("xorwow", "philox", "sobol"), default="philox",
+                    help="generator of the timed calls (default: Philox4x32-10)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (multi-GPU sharding is a design "
@@ -150,22 +167,9 @@ def main() -> int:
         if r.status != 1 or r.c.claimed < REQUIRED:
             print(f"bench.py: bad result {r}", file=sys.stderr)
             return 1
-    dual, t_dual = timed("dual", REQUIRED, a.dual_steps, 1)
-
     conv_adj = sum(r.c.claimed for r in adj)
     value = conv_adj / t_adj
     ms_adj = 1e3 * t_adj / len(adj)
-    ms_dual = 1e3 * t_dual / len(dual)
-    value_dual = sum(r.c.claimed for r in dual) / t_dual
-    # the same workload under each generator: the call time hardly changes,
-    # the converged count at the early stop does (it depends on how tightly
-    # the PSO result's iteration counts cluster)
-    by_gen = {}
-    for g in (gen if a.generators else ()):
-        rs, tg = timed("adjoint", REQUIRED, max(3, a.steps // 4), 1, g)
-        by_gen[g] = {"value": sum(r.c.claimed for r in rs) / tg,
-                     "ms_per_step": 1e3 * tg / len(rs),
-                     "converged_per_call": float(np.mean([r.c.claimed for r in rs]))}
 
     # roofline of the BFGS kernel (device time from events on its stream)
     form, threads = z.tuned_config(capi.OBJ_GAUSSIAN, D, capi.GRAD_ADJOINT)
@@ -178,6 +182,28 @@ def main() -> int:
     else:
         kernel = f"bfgs_group_kernel<Gaussian<150>, {threads} threads, adjoint>"
         layout = "single GPU, one workgroup per start"
+
+    # the same workload under the other generators: the call time hardly
+    # changes, the converged count at the early stop does (it depends on how
+    # tightly the PSO result's iteration counts cluster)
+    others = [g for g in gen if g != a.prng and (a.generators or g != "sobol")]
+    by_gen = {}
+    for g in others:
+        rs, tg = timed("adjoint", REQUIRED, max(3, a.steps // 4), 1, g)
+        by_gen[g] = {"value": sum(r.c.claimed for r in rs) / tg,
+                     "ms_per_step": 1e3 * tg / len(rs),
+                     "converged_per_call": float(np.mean([r.c.claimed for r in rs]))}
+    # no early stop (required = N + 1): every start runs to convergence
+    thr, t_thr = timed("adjoint", N + 1, max(2, a.steps // 4), 1)
+    thr_ms_opt = float(np.mean([r.ms_opt for r in thr]))
+    no_stop = {"value": sum(r.c.claimed for r in thr) / t_thr,
+               "ms_per_step": 1e3 * t_thr / len(thr),
+               "converged_per_call": thr[0].c.claimed,
+               "bfgs_ms": thr_ms_opt,
+               "bfgs_tflops": bfgs_flops(thr, form) / len(thr) / (thr_ms_opt * 1e-3) / 1e12}
+    dual, t_dual = timed("dual", REQUIRED, a.dual_steps, 1)
+    ms_dual = 1e3 * t_dual / len(dual)
+    value_dual = sum(r.c.claimed for r in dual) / t_dual
 
     out = {
         "metric": METRIC,
@@ -208,19 +234,13 @@ def main() -> int:
                  "converged_per_call": dual[0].c.claimed},
         "phases_ms": {"pso": float(np.mean([r.ms_pso for r in adj])), "bfgs": ms_opt},
         "converged_per_call": adj[0].c.claimed,
-        **({"by_generator": by_gen} if by_gen else {}),
+        "by_generator": by_gen,
+        "no_early_stop": no_stop,
         "best_fval": adj[0].fval,
     }
-    if a.throughput:
-        thr, t_thr = timed("adjoint", N + 1, max(2, a.steps // 4), 1)
-        thr_ms_opt = float(np.mean([r.ms_opt for r in thr]))
-        out["no_early_stop"] = {
-            "value": sum(r.c.claimed for r in thr) / t_thr,
-            "ms_per_step": 1e3 * t_thr / len(thr),
-            "bfgs_tflops": bfgs_flops(thr, form) / len(thr) / (thr_ms_opt * 1e-3) / 1e12}
     if not a.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        out["cpu_baseline"] = cpu_baseline(max(1, threads))
+        out["cpu_baseline"] = cpu_baseline(max(1, threads), int(gen[a.prng]))
     print(json.dumps(out), flush=True)
     return 0
 
