@@ -37,6 +37,8 @@ typedef int32_t zeus_status_t;
 #define ZEUS_E_UNSUPPORTED (-11) /* objective/dimension not instantiated */
 #define ZEUS_E_NODEVICE (-12)   /* no usable gfx950 device               */
 #define ZEUS_E_IO (-13)         /* a file could not be written           */
+#define ZEUS_E_SCRATCH (-14)    /* request exceeds the fixed scratch of an
+                                   allocation-free entry point            */
 
 /* per-start / per-call status codes (reference README.md:27-35) */
 #define ZEUS_UNSET (-1)
@@ -225,6 +227,23 @@ zeus_status_t zeus_run_batch(const zeus_problem* probs, int32_t n_probs,
 zeus_status_t zeus_pso_run(const zeus_problem* prob, double* pbest_x,
                            double* pbest_val, double* gbest_x,
                            double* gbest_val, void* stream);
+
+/* zeus_pso_run without any allocation: no counterpart in the reference.  For
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+   kernels and results (bit for bit) as zeus_pso_run; every buffer beyond
+   the caller's outputs (positions, velocities, per-workgroup partials, the
+   Gaussian's prepared matrix, XORWOW states, the control block) lives in a
+   fixed 32 MiB device array of the library, one per GPU, held until the
+   process ends.  A problem whose buffers exceed it returns ZEUS_E_SCRATCH
+   (up to about 0.8e6 start-coordinates with XORWOW draws and 2e6 with
+   PHILOX / SOBOL: 4096 starts x 150 dimensions fit under every generator).  The array keeps the last call's
+   contents; calls are serialized by the library and each returns after its
+   stream has drained, so two calls in one process never overlap (calls that
+   overlapped on one GPU would race on the array). */
+zeus_status_t zeus_pso_run_noalloc(const zeus_problem* prob, double* pbest_x,
+                                   double* pbest_val, double* gbest_x,
+                                   double* gbest_val, void* stream);
 
 /* The swarm of one problem, caller-owned device buffers: positions and
    velocities (N x D, start-major), pBest coordinates and values (N x D, N),

@@ -11,7 +11,8 @@ import os
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libzeus_amd.so")
 
-OK, E_ALLOC, E_LAUNCH, E_ARG, E_UNSUPPORTED, E_NODEVICE, E_IO = 0, 3, 4, -10, -11, -12, -13
+OK, E_ALLOC, E_LAUNCH, E_ARG, E_UNSUPPORTED, E_NODEVICE, E_IO, E_SCRATCH = \
+    0, 3, 4, -10, -11, -12, -13, -14
 UNSET, SURRENDERED, CONVERGED, STOPPED, MALLOC, KERNEL, NONFINITE = -1, 0, 1, 2, 3, 4, 5
 OBJ_ROSENBROCK, OBJ_RASTRIGIN, OBJ_ACKLEY, OBJ_GAUSSIAN, OBJ_LOGLIK, OBJ_MLP, OBJ_SPHERE = (
     1, 2, 3, 4, 5, 6, 7)
@@ -94,6 +95,8 @@ SIGNATURES = {
                                           C.c_int32, C.c_int32, C.c_int32]),
     "zeus_pso_run": (C.c_int32, [C.POINTER(Problem), C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p]),
+    "zeus_pso_run_noalloc": (C.c_int32, [C.POINTER(Problem), C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p]),
     "zeus_eval": (C.c_int32, [C.POINTER(Problem), C.c_void_p, C.c_int64, C.c_void_p,
                               C.c_void_p, C.c_int32, C.c_void_p]),
     "zeus_argmin": (C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]),

@@ -1246,9 +1246,141 @@ pso_phase(const zeus_problem* prob, int t0, int t1, double* X, double* V, double
   return to_status(e != hipSuccess ? e : e2);
 }
 
+// ---- fixed device scratch of the allocation-free PSO entry point ----------
+// One array per GPU the library runs on, held until the process ends.  Its
+// name is a host-side stand-in: host code uses only the address that
+// hipGetSymbolAddress returns for the current device (scratch_base).
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's scratch array (resolved once per device).
+zeus_status_t
+scratch_base(void** out)
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return ZEUS_E_NODEVICE;
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+      return ZEUS_E_LAUNCH;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ZEUS_OK;
+}
+
+// Writes a problem's descriptor and its all-zero workgroup -> problem map
+// into device memory (no host staging copy).
+__global__ void
+noalloc_setup_kernel(ProblemDesc d, ProblemDesc* out, int32_t* block_problem, int n_blocks)
+{
+  const int i = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+  if (i == 0)
+    *out = d;
+  if (i < n_blocks)
+    block_problem[i] = 0;
+}
+
+// zeus_pso_run on the scratch array: the same kernels in the same order on
+// the same inputs, with every buffer beyond the caller's outputs carved from
+// `base` (kScratchBytes) instead of the library's arena.
+zeus_status_t
+pso_run_noalloc(const zeus_problem* prob, double* pbx, double* pbv, double* gbx_out,
+                double* gbv, void* base, hipStream_t st)
+{
+  const ClassOps* ops = rt::find_class(prob->objective, prob->dim);
+  const int D = prob->dim;
+  const int64_t n = prob->n_starts;
+  const int nb = int((n + ops->pso_ppb - 1) / ops->pso_ppb);
+  const int T = prob->pso_iter;
+  const bool gauss = ops->objective == obj::kGaussian;
+  const bool xorwow = prob->prng == ZEUS_PRNG_XORWOW && T > 0;
+
+  Carver cv{static_cast<char*>(base)};
+  ProblemDesc* d_desc = cv.take<ProblemDesc>(1);
+  int32_t* d_bp = cv.take<int32_t>(size_t(nb));
+  double* X = cv.take<double>(size_t(n) * D);
+  double* V = cv.take<double>(size_t(n) * D);
+  double* gbx = gbx_out ? gbx_out : cv.take<double>(size_t(D));
+  double* part_val = cv.take<double>(2 * size_t(nb));
+  int64_t* part_idx = cv.take<int64_t>(2 * size_t(nb));
+  double* part_x = cv.take<double>(2 * size_t(nb) * D);
+  double* gstate = cv.take<double>(2 * size_t(D + 1));
+  double* aux = gauss ? cv.take<double>(size_t(dev::gaussian_aux_doubles(D))) : nullptr;
+  uint32_t* xw = xorwow ? cv.take<uint32_t>(6 * size_t(n) * D) : nullptr;
+  Control* ctl = cv.take<Control>(1);
+  if (cv.off > kScratchBytes)
+    return ZEUS_E_SCRATCH;
+
+  ProblemDesc d{};
+  d.objective = prob->objective;
+  d.dim = D;
+  d.lower = prob->lower;
+  d.upper = prob->upper;
+  d.n_starts = n;
+  d.pso_iter = T;
+  d.seed = prob->seed;
+  d.stream = prob->stream;
+  d.prng = prob->prng;
+  d.params = prob->params;
+  d.n_params = prob->n_params;
+  if (gauss) {
+    d.aux = aux;
+    d.aux_frag = aux + int64_t(D) * D;
+  }
+  noalloc_setup_kernel<<<(std::max(nb, 1) + 255) / 256, 256, 0, st>>>(d, d_desc, d_bp, nb);
+  dev::init_controls_kernel<<<1, 256, 0, st>>>(ctl, 1, gbx, D);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && gauss) {
+    const int64_t na = dev::gaussian_aux_doubles(D);
+    dev::symmetrize_kernel<<<int((na + 255) / 256), 256, 0, st>>>(d_desc, 0, aux,
+                                                                  aux + int64_t(D) * D, D);
+    e = hipGetLastError();
+  }
+  PsoArgs pa{d_desc, 1,    0,        nb,       ctl,    X,    V,         pbx, pbv,
+             gbx,    part_val, part_idx, part_x, gstate, d_bp, xw, n * D, 0};
+  for (int it = 0; e == hipSuccess && it <= T; ++it) {
+    pa.iter = it;
+    e = ops->pso(pa, nb, st);
+  }
+  if (e == hipSuccess)
+    e = ops->pso_final(pa, 1, st);
+  if (e == hipSuccess && gbv)
+    e = hipMemcpyAsync(gbv, &ctl->gbest_val, sizeof(double), hipMemcpyDeviceToDevice, st);
+  const hipError_t e2 = hipStreamSynchronize(st);
+  return to_status(e != hipSuccess ? e : e2);
+}
+
 } // namespace
 
 extern "C" {
+
+zeus_status_t
+zeus_pso_run_noalloc(const zeus_problem* prob, double* pbest_x, double* pbest_val,
+                     double* gbest_x, double* gbest_val, void* stream)
+{
+  if (!prob || !pbest_x || !pbest_val)
+    return ZEUS_E_ARG;
+  zeus_status_t rc = validate(*prob);
+  if (rc != ZEUS_OK)
+    return rc;
+  if (prob->pso_iter < 0)
+    return ZEUS_E_ARG;
+  if (prob->n_starts == 0)
+    return ZEUS_OK;
+  std::lock_guard<std::mutex> lock(g_mu);
+  if ((rc = ensure_device()) != ZEUS_OK)
+    return rc;
+  void* base = nullptr;
+  if ((rc = scratch_base(&base)) != ZEUS_OK)
+    return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g_ds.stream;
+  return pso_run_noalloc(prob, pbest_x, pbest_val, gbest_x, gbest_val, base, st);
+}
 
 zeus_status_t
 zeus_pso_run(const zeus_problem* prob, double* pbest_x, double* pbest_val, double* gbest_x,
@@ -1506,6 +1638,7 @@ zeus_status_string(int32_t status)
     case ZEUS_E_UNSUPPORTED: return "objective/dimension not instantiated";
     case ZEUS_E_NODEVICE: return "no gfx950 device";
     case ZEUS_E_IO: return "file could not be written";
+    case ZEUS_E_SCRATCH: return "request exceeds the fixed scratch of the allocation-free entry point";
     default: return "unknown";
   }
 }
