@@ -12,6 +12,7 @@
 
 #include "matrix.hpp"
 #include "objectives.hpp"
+#include "zeus_c.h"
 
 #include <array>
 #include <cmath>
@@ -22,6 +23,11 @@
 class LogLikelihood {
 public:
   static constexpr std::size_t arity = 4;
+  // a built-in objective (zeus/builtins.hpp): its parameter block is the
+  // library's ZEUS_OBJ_LOGLIK layout [x (n) | k (n) | lgamma(k + 1) (n)]
+  static constexpr int zeus_objective = ZEUS_OBJ_LOGLIK;
+  const double* zeus_params() const { return P_.deviceData(); }
+  long long zeus_n_params() const { return static_cast<long long>(3 * n_); }
 
   LogLikelihood(std::vector<int> const& bin_counts, std::vector<double> const& bin_centers)
     : n_(bin_counts.size()), P_(3 * bin_counts.size(), 1)

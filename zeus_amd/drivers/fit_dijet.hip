@@ -1,192 +1,218 @@
-// fit_dijet.hip -- the reference's dijet-spectrum fit driver
-// (examples/fit_dijet_spectrum.cu:19-221): load a three-column TSV
-// (bin_low bin_high counts, one header line), convert bin centres to
-// x = m / sqrt_s, fit the four-parameter Poisson likelihood over [0, 10]^4
-// with 10 PSO iterations, 100 required convergences, tol 1e-8, seed 42, and
-// report the post-fit chi2/ndf and deviance/ndf; per-bin predictions go to
-// dijet_fit_vs_data.tsv.
+// fit_dijet.hip -- dijet-spectrum fit program.
+//
+// Fits the four-parameter dijet shape mu(x) = a (1 - x)^b / x^(c + d ln x)
+// to a binned mass spectrum by minimising the Poisson negative
+// log-likelihood with zeus::Zeus over the box [0, 10]^4 (10 PSO iterations,
+// 100 required convergences, tol 1e-8, seed 42), then reports the fit
+// quality and writes the per-bin prediction table.  Command line, printed
+// lines and the table's columns are those of the reference program
+// (examples/fit_dijet_spectrum.cu:19-221), which scripts parse:
 //
 //   zeus_fit_dijet <num_optimizations> <max_bfgs_iter> <run_id> [--parallel]
 //                  [--save-trajectories <filename>] [--prng <xorwow|philox|sobol>]
-//                  [--nzerosteps <n>]
-//                  [--data <path>] [--sqrt-s <units>] [--adjoint]
+//                  [--nzerosteps <n>] [--data <path>] [--sqrt-s <units>] [--adjoint]
 //
-// --data (default: the reference's ../examples/simulation/dijet_spectrum.tsv,
-// which the reference does not ship), --sqrt-s (default 13.6, the
-// reference's value) and --adjoint are this build's options.
+// Input: a text table with one header line and rows "bin_low bin_high
+// counts"; a bin is placed at x = (bin_low + bin_high) / (2 sqrt_s), kept
+// inside (0, 1) by 1e-12.  --data (default ../examples/simulation/
+// dijet_spectrum.tsv), --sqrt-s (default 13.6) and --adjoint (hand-written
+// adjoint gradient instead of dual numbers) are this program's options.
+// LogLikelihood is a built-in objective: with --parallel and no trajectory
+// the fit runs on the library's kernels for it.
 #include "zeus/loglikelihood.hpp"
 #include "zeus/zeus.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
-#include <fstream>
-#include <iostream>
-#include <stdexcept>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
 namespace {
 
-void
-load_dijet_spectrum_as_x(const std::string& path, std::vector<double>& bin_low,
-                         std::vector<double>& bin_high, std::vector<int>& counts,
-                         std::vector<double>& x_centers, double sqrt_s_units)
+constexpr double kEdge = 1e-12; // distance of x from 0 and 1, floor of mu
+
+// A binned spectrum: edges, counts and the position x of each bin.
+struct Spectrum {
+  std::vector<double> lo, hi, x;
+  std::vector<int> k;
+
+  long long events() const
+  {
+    long long t = 0;
+    for (int v : k)
+      t += v;
+    return t;
+  }
+};
+
+// Reads the table at `path`; on failure returns false with `why` set.
+bool
+read_spectrum(const std::string& path, double sqrt_s, Spectrum& sp, std::string& why)
 {
-  bin_low.clear();
-  bin_high.clear();
-  counts.clear();
-  x_centers.clear();
-  std::ifstream in(path);
-  if (!in)
-    throw std::runtime_error("couldn't open: " + path);
-  std::string header;
-  std::getline(in, header);
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) {
+    why = "couldn't open: " + path;
+    return false;
+  }
+  int c;
+  while ((c = std::fgetc(f)) != EOF && c != '\n') // header line
+    ;
   double lo, hi;
   int k;
-  constexpr double eps = 1e-12;
-  while (in >> lo >> hi >> k) {
-    const double x = std::min(std::max(0.5 * (lo + hi) / sqrt_s_units, eps), 1.0 - eps);
-    bin_low.push_back(lo);
-    bin_high.push_back(hi);
-    counts.push_back(k);
-    x_centers.push_back(x);
+  while (std::fscanf(f, "%lf %lf %d", &lo, &hi, &k) == 3) {
+    sp.lo.push_back(lo);
+    sp.hi.push_back(hi);
+    sp.k.push_back(k);
+    sp.x.push_back(std::clamp(0.5 * (lo + hi) / sqrt_s, kEdge, 1.0 - kEdge));
   }
-  if (counts.empty())
-    throw std::runtime_error("empty or mismatched data in: " + path);
-  const auto [mn, mx] = std::minmax_element(x_centers.begin(), x_centers.end());
-  std::cout << "Loaded " << counts.size() << " bins; x in [" << *mn << ", " << *mx << "]\n";
+  std::fclose(f);
+  if (sp.k.empty()) {
+    why = "empty or mismatched data in: " + path;
+    return false;
+  }
+  return true;
 }
 
-void
-write_pred_vs_true_tsv(const std::string& path, const std::vector<double>& bin_low,
-                       const std::vector<double>& bin_high, const std::vector<int>& counts,
-                       const std::vector<double>& x, const std::vector<double>& mu,
-                       const std::vector<double>& residual, const std::vector<double>& pull)
+// The fitted model at every bin and the two goodness-of-fit sums.
+struct Prediction {
+  std::vector<double> mu, residual, pull;
+  double chi2 = 0.0, deviance = 0.0;
+};
+
+Prediction
+predict(const Spectrum& sp, const double* p)
 {
-  FILE* out = std::fopen(path.c_str(), "w");
-  if (!out)
-    throw std::runtime_error("could not open: " + path);
-  std::fputs("bin_low\tbin_high\tx_center\ttrue_counts\tpred_mu\tpred_residual\tpred_pull\n",
-             out);
-  for (std::size_t i = 0; i < counts.size(); ++i)
-    std::fprintf(out, "%.17g\t%.17g\t%.17g\t%d\t%.17g\t%.17g\t%.17g\n", bin_low[i], bin_high[i],
-                 x[i], counts[i], mu[i], residual[i], pull[i]);
-  std::fclose(out);
+  Prediction pr;
+  const std::size_t n = sp.k.size();
+  pr.mu.resize(n);
+  pr.residual.resize(n);
+  pr.pull.resize(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    const double x = sp.x[i], k = double(sp.k[i]);
+    const double mu = p[0] * std::pow(1.0 - x, p[1]) / std::pow(x, p[2] + p[3] * std::log(x));
+    const double m = std::max(mu, kEdge);
+    pr.mu[i] = mu;
+    pr.residual[i] = k - mu;
+    pr.pull[i] = pr.residual[i] / std::sqrt(m);
+    pr.chi2 += (k - m) * (k - m) / m;
+    pr.deviance += k > 0 ? 2.0 * (k * std::log(k / m) - (k - m)) : 2.0 * m;
+  }
+  return pr;
 }
 
+bool
+write_table(const char* path, const Spectrum& sp, const Prediction& pr)
+{
+  FILE* f = std::fopen(path, "w");
+  if (!f)
+    return false;
+  std::fputs("bin_low\tbin_high\tx_center\ttrue_counts\tpred_mu\tpred_residual\tpred_pull\n", f);
+  for (std::size_t i = 0; i < sp.k.size(); ++i)
+    std::fprintf(f, "%.17g\t%.17g\t%.17g\t%d\t%.17g\t%.17g\t%.17g\n", sp.lo[i], sp.hi[i],
+                 sp.x[i], sp.k[i], pr.mu[i], pr.residual[i], pr.pull[i]);
+  return std::fclose(f) == 0;
+}
+
+struct Cli {
+  std::size_t starts = 0;
+  int max_iter = 0, run = 0, nzerosteps = 0;
+  bool parallel = false, adjoint = false;
+  zeus::PRNGType prng = zeus::PRNGType::XORWOW;
+  std::string trajectory, data = "../examples/simulation/dijet_spectrum.tsv";
+  double sqrt_s = 13.6;
+};
+
+// 0: parsed; 1: bad command line (the message is printed here).
 int
-usage(const char* prog)
+parse(int argc, char** argv, Cli& o)
 {
-  std::cerr << "Usage: " << prog
-            << " <num_optimizations> <max_bfgs_iter> <run_id> "
-               "[--parallel] [--save-trajectories <filename>] [--prng <xorwow|philox|sobol>]"
-               " [--nzerosteps <n>] [--data <path>] [--sqrt-s <units>] [--adjoint]\n";
-  return 1;
+  const char* usage = " <num_optimizations> <max_bfgs_iter> <run_id> [--parallel] "
+                      "[--save-trajectories <filename>] [--prng <xorwow|philox|sobol>] "
+                      "[--nzerosteps <n>] [--data <path>] [--sqrt-s <units>] [--adjoint]\n";
+  // the three positional numbers: whole decimal integers, N >= 0
+  long long num[3] = {0, 0, 0};
+  bool ok = argc >= 4;
+  for (int j = 0; ok && j < 3; ++j) {
+    char* end = nullptr;
+    num[j] = std::strtoll(argv[1 + j], &end, 10);
+    ok = end != argv[1 + j] && *end == '\0';
+  }
+  if (!ok || num[0] < 0) {
+    std::fprintf(stderr, "Usage: %s%s", argv[0], usage);
+    return 1;
+  }
+  o.starts = std::size_t(num[0]);
+  o.max_iter = int(num[1]);
+  o.run = int(num[2]);
+  for (int i = 4; i < argc; ++i) {
+    const std::string a = argv[i];
+    const bool more = i + 1 < argc;
+    if (a == "--parallel") {
+      o.parallel = true;
+    } else if (a == "--adjoint") {
+      o.adjoint = true;
+    } else if (a == "--save-trajectories" && more) {
+      o.trajectory = argv[++i];
+    } else if (a == "--nzerosteps" && more) {
+      o.nzerosteps = std::atoi(argv[++i]);
+    } else if (a == "--data" && more) {
+      o.data = argv[++i];
+    } else if (a == "--sqrt-s" && more) {
+      o.sqrt_s = std::atof(argv[++i]);
+    } else if (a == "--prng" && more) {
+      const std::string v = argv[++i];
+      if (v == "philox")
+        o.prng = zeus::PRNGType::PHILOX;
+      else if (v == "sobol")
+        o.prng = zeus::PRNGType::SOBOL;
+      else if (v != "xorwow")
+        std::fprintf(stderr, "Unknown PRNG type: %s. Using default xorwow.\n", v.c_str());
+    } else {
+      std::fprintf(stderr, "Unknown argument: %s\n", a.c_str());
+      return 1;
+    }
+  }
+  return 0;
 }
 
 } // namespace
 
 int
-main(int argc, char* argv[])
+main(int argc, char** argv)
 {
-  if (argc < 4)
-    return usage(argv[0]);
-  std::size_t N = 0;
-  int bfgs = 0, run = 0;
-  try {
-    N = std::stoul(argv[1]);
-    bfgs = std::stoi(argv[2]);
-    run = std::stoi(argv[3]);
-  } catch (const std::exception&) {
-    return usage(argv[0]);
-  }
-  std::string trajectory_file, data = "../examples/simulation/dijet_spectrum.tsv";
-  double sqrt_s = 13.6;
-  zeus::PRNGType prng = zeus::PRNGType::XORWOW;
-  bool parallel = false, adjoint = false;
-  int nzerosteps = 0;
-  for (int i = 4; i < argc; ++i) {
-    const std::string a = argv[i];
-    if (a == "--parallel") {
-      parallel = true;
-    } else if (a == "--save-trajectories" && i + 1 < argc) {
-      trajectory_file = argv[++i];
-    } else if (a == "--prng" && i + 1 < argc) {
-      const std::string v = argv[++i];
-      if (v == "xorwow")
-        prng = zeus::PRNGType::XORWOW;
-      else if (v == "philox")
-        prng = zeus::PRNGType::PHILOX;
-      else if (v == "sobol")
-        prng = zeus::PRNGType::SOBOL;
-      else
-        std::cerr << "Unknown PRNG type: " << v << ". Using default xorwow.\n";
-    } else if (a == "--nzerosteps" && i + 1 < argc) {
-      nzerosteps = std::stoi(argv[++i]);
-    } else if (a == "--data" && i + 1 < argc) {
-      data = argv[++i];
-    } else if (a == "--sqrt-s" && i + 1 < argc) {
-      sqrt_s = std::stod(argv[++i]);
-    } else if (a == "--adjoint") {
-      adjoint = true;
-    } else {
-      std::cerr << "Unknown argument: " << a << "\n";
-      return 1;
-    }
-  }
+  Cli o;
+  if (parse(argc, argv, o) != 0)
+    return 1;
   util::setStackSize();
 
-  std::vector<double> bin_low, bin_high, centers;
-  std::vector<int> counts;
-  try {
-    load_dijet_spectrum_as_x(data, bin_low, bin_high, counts, centers, sqrt_s);
-  } catch (const std::exception& e) {
-    std::cerr << e.what() << "\n";
+  Spectrum sp;
+  std::string why;
+  if (!read_spectrum(o.data, o.sqrt_s, sp, why)) {
+    std::fprintf(stderr, "%s\n", why.c_str());
     return 1;
   }
-  std::cout << "\n#bins = " << counts.size() << "\n";
-  long long total = 0;
-  for (int k : counts)
-    total += k;
-  std::cout << "total events in the spectrum = " << total << "\n\n";
+  const auto [xmin, xmax] = std::minmax_element(sp.x.begin(), sp.x.end());
+  std::printf("Loaded %zu bins; x in [%g, %g]\n", sp.k.size(), *xmin, *xmax);
+  std::printf("\n#bins = %zu\ntotal events in the spectrum = %lld\n\n", sp.k.size(),
+              sp.events());
 
-  LogLikelihood ll(counts, centers);
-  const auto g = adjoint ? zeus::Gradient::Adjoint : zeus::Gradient::Dual;
-  auto res = zeus::Zeus(ll, 0.00, 10.00, N, bfgs, 10, 100, "poisson", 1e-8, 42, run, parallel,
-                        prng, trajectory_file, nzerosteps, g);
-  std::cout << "best NLL: " << res.fval << "\n";
-  const double a = res.coordinates[0], b = res.coordinates[1], c = res.coordinates[2],
-               d = res.coordinates[3];
-  std::cout << "best-fit parameters: a=" << a << "  b=" << b << "  c=" << c << "  d=" << d
-            << "\n\n";
+  const LogLikelihood nll(sp.k, sp.x);
+  const auto r = zeus::Zeus(nll, 0.0, 10.0, o.starts, o.max_iter, 10, 100, "poisson", 1e-8, 42,
+                            o.run, o.parallel, o.prng, o.trajectory, o.nzerosteps,
+                            o.adjoint ? zeus::Gradient::Adjoint : zeus::Gradient::Dual);
+  const double* p = r.coordinates;
+  std::printf("best NLL: %g\n", r.fval);
+  std::printf("best-fit parameters: a=%g  b=%g  c=%g  d=%g\n\n", p[0], p[1], p[2], p[3]);
 
-  // post-fit diagnostics (fit_dijet_spectrum.cu:189-214)
-  constexpr double eps = 1e-12;
-  const std::size_t n = centers.size();
-  std::vector<double> mu(n), residual(n), pull(n);
-  for (std::size_t i = 0; i < n; ++i) {
-    const double x = std::min(std::max(centers[i], eps), 1.0 - eps);
-    mu[i] = a * std::pow(1.0 - x, b) / std::pow(x, c + d * std::log(x));
-    residual[i] = double(counts[i]) - mu[i];
-    pull[i] = residual[i] / std::sqrt(std::max(mu[i], eps));
-  }
-  double chi2 = 0.0, dev = 0.0;
-  for (std::size_t i = 0; i < n; ++i) {
-    const double k = double(counts[i]);
-    const double m = std::max(mu[i], eps);
-    chi2 += (k - m) * (k - m) / m;
-    dev += (k > 0) ? 2.0 * (k * std::log(k / m) - (k - m)) : 2.0 * m;
-  }
-  const int ndf = std::max<int>(1, int(n) - 4);
-  std::cout << "chi2/ndf=" << chi2 / ndf << "  dev/ndf=" << dev / ndf << "\n";
-  try {
-    write_pred_vs_true_tsv("dijet_fit_vs_data.tsv", bin_low, bin_high, counts, centers, mu,
-                           residual, pull);
-  } catch (const std::exception& e) {
-    std::cerr << e.what() << "\n";
+  const Prediction pr = predict(sp, p);
+  const int ndf = std::max(1, int(sp.k.size()) - 4);
+  std::printf("chi2/ndf=%g  dev/ndf=%g\n", pr.chi2 / ndf, pr.deviance / ndf);
+  if (!write_table("dijet_fit_vs_data.tsv", sp, pr)) {
+    std::fprintf(stderr, "could not open: dijet_fit_vs_data.tsv\n");
     return 1;
   }
-  std::cout << "predictions written to dijet_fit_vs_data.tsv\n";
+  std::printf("predictions written to dijet_fit_vs_data.tsv\n");
   return 0;
 }
