@@ -11,7 +11,9 @@ before the timed region (inputs resident in HBM).
 Also measured on the same GPU, after the timed headline calls (so that the
 first warmup + steps launches of the BFGS kernel in a run are the headline
 ones): the same call under the XORWOW generator (the reference's default),
-the no-early-stop throughput (required = N + 1), the dual-number gradient on
+the no-early-stop throughput (required = N + 1), the call replayed from a
+captured hipGraph, the call through the C++ front end
+(zeus_amd/bin/zeus_gaussian_bench), the dual-number gradient on
 the same call (the t_dual / t_adjoint ratio), and the CPU oracle on a
 bounded sample of the same workload in two modes (the reference-literal
 O(D^4) update and the rank-2 O(D^2) update, dual gradient both).
@@ -24,6 +26,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -149,16 +152,16 @@ def main() -> int:
     gen = {"xorwow": z.PRNGType.XORWOW, "philox": z.PRNGType.PHILOX,
            "sobol": z.PRNGType.SOBOL}
 
-    def call(grad, required, prng=a.prng):
+    def call(grad, required, prng=a.prng, graph=False):
         return z.Zeus(f, -5.0, 5.0, N, MAX_ITER, PSO_ITER, required, "gaussian", TOL, SEED, 0,
-                      prng=gen[prng], gradient=grad)
+                      prng=gen[prng], gradient=grad, use_graph=graph)
 
-    def timed(grad, required, steps, warmup, prng=a.prng):
+    def timed(grad, required, steps, warmup, prng=a.prng, graph=False):
         for _ in range(warmup):
-            call(grad, required, prng)
+            call(grad, required, prng, graph)
         capi.check(capi.lib().zeus_synchronize(), "sync")
         t0 = time.perf_counter()
-        res = [call(grad, required, prng) for _ in range(steps)]
+        res = [call(grad, required, prng, graph) for _ in range(steps)]
         capi.check(capi.lib().zeus_synchronize(), "sync")
         return res, time.perf_counter() - t0
 
@@ -201,6 +204,19 @@ def main() -> int:
                "converged_per_call": thr[0].c.claimed,
                "bfgs_ms": thr_ms_opt,
                "bfgs_tflops": bfgs_flops(thr, form) / len(thr) / (thr_ms_opt * 1e-3) / 1e12}
+    # the same call replayed from a captured hipGraph (shapes settled above)
+    gr, t_gr = timed("adjoint", REQUIRED, max(3, a.steps // 2), 2, graph=True)
+    graph = {"value": sum(r.c.claimed for r in gr) / t_gr, "ms_per_step": 1e3 * t_gr / len(gr),
+             "converged_per_call": float(np.mean([r.c.claimed for r in gr])),
+             "device_ms_per_call": float(np.mean([r.ms_opt for r in gr]))}
+    # the same call through the C++ front end (zeus::Zeus(zeus::Gaussian<150>{C}, ...))
+    cpp = None
+    exe = os.path.join(ROOT, "zeus_amd", "bin", "zeus_gaussian_bench")
+    if os.path.exists(exe):
+        p = subprocess.run([exe, str(max(3, a.steps // 2)), "3", a.prng], capture_output=True,
+                           text=True, timeout=300)
+        if p.returncode == 0 and p.stdout.strip():
+            cpp = json.loads(p.stdout.strip().splitlines()[-1])
     dual, t_dual = timed("dual", REQUIRED, a.dual_steps, 1)
     ms_dual = 1e3 * t_dual / len(dual)
     value_dual = sum(r.c.claimed for r in dual) / t_dual
@@ -236,6 +252,8 @@ def main() -> int:
         "converged_per_call": adj[0].c.claimed,
         "by_generator": by_gen,
         "no_early_stop": no_stop,
+        "graph": graph,
+        "cpp_frontend": cpp,
         "best_fval": adj[0].fval,
     }
     if not a.no_cpu:

@@ -57,6 +57,9 @@ struct PairLds {
   // stop[h % 2] and read by the logic of half-step h + 1, so a write never
   // meets a read of the same word within a half-step
   int stop[2];
+  // next product unit to take in half-step h (index h % 2); reset in
+  // half-step h - 1, after its last use in h - 2
+  int next_unit[2];
   TileSlot slot[kTileStarts];
   ProblemDesc prob;
 };
@@ -415,8 +418,10 @@ bfgs_tile_paired_kernel(TileArgs a)
         L.alpha[w][1] = 0.0;
         L.live[0][w] = L.live[1][w] = has;
       }
-      if (tid == 0)
+      if (tid == 0) {
         L.stop[0] = L.stop[1] = 0;
+        L.next_unit[0] = L.next_unit[1] = 0;
+      }
     }
     Control* ctl = a.ctl + pi;
     __syncthreads();
@@ -427,41 +432,7 @@ bfgs_tile_paired_kernel(TileArgs a)
       int conv = 0;
       if (tid == kStopReader)
         conv = __hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (my_half == ph) {
-        // ---- product of half-tile ph: units lw, lw + 8, lw + 16 ----
-        int any = 0;
-#pragma unroll
-        for (int j = 0; j < kPairStarts; ++j)
-          any |= L.live[(h + 1) & 1][ph * kPairStarts + j];
-        if (any) {
-          const int sj = ph * kPairStarts + li / 2; // start of column li
-          const double al = L.alpha[sj][li & 1];
-          const double* xs = L.x + sj * VS;
-          const double* ps = L.p + sj * VS;
-          double* gt = L.gt[ph];
-          double* fp_out = L.fpart[ph];
-          for (int u = lw; u < S::U; u += kPairWaves) {
-            const int mb = u >> 1, kh = u & 1;
-            const mfma_acc_t acc =
-              kh ? pair_unit_range<D, S::KB1, S::KB>(Cf, xs, ps, al, mb, li, lk)
-                 : pair_unit_range<D, 0, S::KB1>(Cf, xs, ps, al, mb, li, lk);
-            double* gp = gt + kh * D * S::XS;
-            double fp = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int rr = 16 * mb + 4 * q + lk;
-              if (rr < D) {
-                gp[rr * S::XS + li] = acc[q];
-                fp = ::fma(::fma(al, ps[rr], xs[rr]), acc[q], fp);
-              }
-            }
-            fp += __shfl_xor(fp, 16, kWave);
-            fp += __shfl_xor(fp, 32, kWave);
-            if (lk == 0)
-              fp_out[u * kCsTile + li] = fp;
-          }
-        }
-      } else if (h > 0) {
+      if (my_half != ph && h > 0) {
         // ---- logic of the other half-tile's start w on its last product ----
         if (L.slot[w].phase != 0) {
           static_assert(S::U <= kWave, "one unit per lane");
@@ -474,6 +445,49 @@ bfgs_tile_paired_kernel(TileArgs a)
           pair_slot_step<D>(L, w, P, ctl, a.out, slot_mem(w), f0, f1, L.stop[(h + 1) & 1]);
         }
       }
+      // ---- product of half-tile ph: every wavefront takes units from a
+      // counter, the other half-tile's wavefronts once their logic is done
+      // (a unit's result does not depend on which wavefront computes it) ----
+      int live_ph = 0;
+#pragma unroll
+      for (int j = 0; j < kPairStarts; ++j)
+        live_ph |= L.live[(h + 1) & 1][ph * kPairStarts + j];
+      if (live_ph) {
+        const int sj = ph * kPairStarts + li / 2; // start of column li
+        const double al = L.alpha[sj][li & 1];
+        const double* xs = L.x + sj * VS;
+        const double* ps = L.p + sj * VS;
+        double* gt = L.gt[ph];
+        double* fp_out = L.fpart[ph];
+        for (;;) {
+          int u = 0;
+          if (lane == 0)
+            u = atomicAdd(&L.next_unit[h & 1], 1);
+          u = __builtin_amdgcn_readfirstlane(u);
+          if (u >= S::U)
+            break;
+          const int mb = u >> 1, kh = u & 1;
+          const mfma_acc_t acc =
+            kh ? pair_unit_range<D, S::KB1, S::KB>(Cf, xs, ps, al, mb, li, lk)
+               : pair_unit_range<D, 0, S::KB1>(Cf, xs, ps, al, mb, li, lk);
+          double* gp = gt + kh * D * S::XS;
+          double fp = 0.0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int rr = 16 * mb + 4 * q + lk;
+            if (rr < D) {
+              gp[rr * S::XS + li] = acc[q];
+              fp = ::fma(::fma(al, ps[rr], xs[rr]), acc[q], fp);
+            }
+          }
+          fp += __shfl_xor(fp, 16, kWave);
+          fp += __shfl_xor(fp, 32, kWave);
+          if (lk == 0)
+            fp_out[u * kCsTile + li] = fp;
+        }
+      }
+      if (tid == 0) // counter of half-step h + 1 (last used in h - 1)
+        L.next_unit[(h + 1) & 1] = 0;
       // every wavefront publishes its own slot's state for this half-step
       if (lane == 0)
         L.live[h & 1][w] = L.slot[w].phase != 0;
