@@ -188,10 +188,11 @@ typedef struct zeus_result {
   double fval;
   double gradient_norm;
   int32_t claimed, actual, surrendered, stopped; /* util.hpp:27-32        */
-  double ms_pso;        /* device time of the PSO phase (0 when replayed
-                           from a graph)                                   */
-  double ms_opt;        /* device time of the BFGS phase (of the whole
-                           replayed graph when use_graph)                  */
+  double ms_pso;        /* device time of the PSO phase of this problem's
+                           class (problems of one objective and dim; 0
+                           when replayed from a graph)                     */
+  double ms_opt;        /* device time of its class's BFGS phase (of the
+                           whole replayed graph when use_graph)            */
   double ms_total;      /* host wall time of the call                      */
   int64_t sum_iter;     /* sum over starts of BFGS iterations              */
   int64_t sum_ls_evals; /* sum over starts of line-search evaluations      */

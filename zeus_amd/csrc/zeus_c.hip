@@ -797,12 +797,13 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     });
     for (size_t r = 0; r < order.size(); ++r)
       slot[order[r]] = int(r % DeviceState::kSide);
-    while (g_ds.cls_ev.size() < 3 * classes.size()) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess)
-        return fail(hipErrorOutOfMemory);
-      g_ds.cls_ev.push_back(e);
-    }
+  }
+  // per class: PSO start, PSO end, BFGS end (classes are timed one by one)
+  while (g_ds.cls_ev.size() < 3 * classes.size()) {
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess)
+      return fail(hipErrorOutOfMemory);
+    g_ds.cls_ev.push_back(ev);
   }
   auto sequence = [&](bool timed) -> hipError_t {
     hipError_t err = hipSuccess;
@@ -869,9 +870,14 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     }
     if (timed)
       (void)hipEventRecord(g_ds.ev[0], st);
-    for (auto& c : classes)
-      if ((err = launch_class(c, d_ctl, opt.gradient, st)) != hipSuccess)
+    for (size_t k = 0; k < classes.size(); ++k) {
+      if (timed)
+        (void)hipEventRecord(g_ds.cls_ev[3 * k], st);
+      if ((err = launch_class(classes[k], d_ctl, opt.gradient, st)) != hipSuccess)
         return err;
+      if (timed)
+        (void)hipEventRecord(g_ds.cls_ev[3 * k + 1], st);
+    }
     // trajectory buffers: NaN / -1 / 0 before the BFGS writes its steps
     for (const auto& c : classes)
       for (const auto& d : c.desc)
@@ -884,9 +890,12 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       return err;
     if (timed)
       (void)hipEventRecord(g_ds.ev[1], st);
-    for (auto& c : classes) {
+    for (size_t k = 0; k < classes.size(); ++k) {
+      auto& c = classes[k];
       if ((err = launch_bfgs(c, d_ctl, opt.gradient, st)) != hipSuccess)
         return err;
+      if (timed)
+        (void)hipEventRecord(g_ds.cls_ev[3 * k + 2], st);
       g_ds.last[{c.ops->objective, c.ops->dim, opt.gradient}] = c.cfg;
     }
     if (timed)
@@ -955,20 +964,32 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   const Control* h_ctl = reinterpret_cast<const Control*>(mirror(d_ctl));
   const double* h_bx = reinterpret_cast<const double*>(mirror(d_bx));
 
+  // device times per class: its PSO phase (objective preparation included)
+  // and its BFGS phase; a replayed graph reports its whole device time
+  std::vector<float> cls_pso(classes.size(), 0.f), cls_opt(classes.size(), 0.f);
   float ms_pso = 0.f, ms_opt = 0.f;
   if (graphed) { // one replay: its device time (PSO, BFGS and summary) in ms_opt
     (void)hipEventElapsedTime(&ms_opt, g_ds.ev[0], g_ds.ev[2]);
-  } else if (concurrent) { // classes side by side: the longest PSO and BFGS phases
-    for (size_t k = 0; k < classes.size(); ++k) {
-      float a = 0.f, b = 0.f;
-      (void)hipEventElapsedTime(&a, g_ds.cls_ev[3 * k], g_ds.cls_ev[3 * k + 1]);
-      (void)hipEventElapsedTime(&b, g_ds.cls_ev[3 * k + 1], g_ds.cls_ev[3 * k + 2]);
-      ms_pso = std::max(ms_pso, a);
-      ms_opt = std::max(ms_opt, b);
-    }
+    std::fill(cls_opt.begin(), cls_opt.end(), ms_opt);
   } else {
-    (void)hipEventElapsedTime(&ms_pso, g_ds.ev[0], g_ds.ev[1]);
-    (void)hipEventElapsedTime(&ms_opt, g_ds.ev[1], g_ds.ev[2]);
+    for (size_t k = 0; k < classes.size(); ++k) {
+      // side by side, a class's BFGS follows its own PSO; one after the other,
+      // it follows the previous class's BFGS (or the end of every PSO phase)
+      const hipEvent_t b0 = concurrent ? g_ds.cls_ev[3 * k + 1]
+                            : k == 0   ? g_ds.ev[1]
+                                       : g_ds.cls_ev[3 * (k - 1) + 2];
+      (void)hipEventElapsedTime(&cls_pso[k], g_ds.cls_ev[3 * k], g_ds.cls_ev[3 * k + 1]);
+      (void)hipEventElapsedTime(&cls_opt[k], b0, g_ds.cls_ev[3 * k + 2]);
+    }
+    if (concurrent) { // the longest PSO and BFGS phases
+      for (size_t k = 0; k < classes.size(); ++k) {
+        ms_pso = std::max(ms_pso, cls_pso[k]);
+        ms_opt = std::max(ms_opt, cls_opt[k]);
+      }
+    } else {
+      (void)hipEventElapsedTime(&ms_pso, g_ds.ev[0], g_ds.ev[1]);
+      (void)hipEventElapsedTime(&ms_opt, g_ds.ev[1], g_ds.ev[2]);
+    }
   }
 
   // per-start outputs, scattered back into problem order
@@ -1059,8 +1080,8 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     r.sum_iter = k.sum_iter;
     r.sum_ls_evals = k.sum_ls_evals;
     r.sum_grads = k.sum_grads;
-    r.ms_pso = ms_pso;
-    r.ms_opt = ms_opt;
+    r.ms_pso = cls_pso[cls_of[i]];
+    r.ms_opt = cls_opt[cls_of[i]];
     r.ms_total = ms_total;
     r.ad = m_ad[i];
     r.bfgs = m_bfgs[i];

@@ -3,7 +3,7 @@ build one workload, run it warm, print one JSON line with its timing and
 counts.
 
     python -m zeus_amd.workloads rosenbrock|dijet|mlp|mixed [--calls K] [--max-iter M]
-                                 [--gradient G]
+                                 [--gradient G] [--graph]
 
 rosenbrock  Rosenbrock D = 2, box [-5, 5], 1024 starts, PSO 10, MAX_ITER 10000,
     required 100, tol 1e-8, seed 42 (the reference's CPU-runnable case).
@@ -17,7 +17,9 @@ mixed  64 problems in one batched call: Rosenbrock D in {2, 10, 50}, Rastrigin
     D = 20, Ackley D = 10, Gaussian D in {32, 150} (C = A A^T / D + I), start
     counts log-uniform in [128, 4096], required min(100, starts); kinds and
     counts drawn from a numpy generator seeded with 42; PSO 10, MAX_ITER
-    10000, tol 1e-8.
+    10000, tol 1e-8.  Reports each class's device times and rates
+    (class_table); --graph replays the settled batch from a captured
+    hipGraph (the per-class times are then not separated).
 
 The first call of a workload is a warm-up (code objects, pools, autotuner);
 the line reports the fastest of the timed calls.  Synthetic data, FP64.
@@ -83,7 +85,7 @@ def _mixed_problems(max_starts=None, max_iter=10000, early_stop=True):
     return probs, names
 
 
-def run(name: str, calls: int, max_iter: int | None, gradient: str) -> dict:
+def run(name: str, calls: int, max_iter: int | None, gradient: str, graph: bool = False) -> dict:
     out = {"workload": name, "gradient": gradient, "data": "synthetic", "dtype": "f64"}
     if name == "rosenbrock":
         def call():
@@ -110,8 +112,8 @@ def run(name: str, calls: int, max_iter: int | None, gradient: str) -> dict:
         probs, names = _mixed_problems()
 
         def call():
-            return z.run_batch(probs, gradient=gradient)
-        out.update(starts=sum(p.N for p in probs), problems=len(probs))
+            return z.run_batch(probs, gradient=gradient, use_graph=graph)
+        out.update(starts=sum(p.N for p in probs), problems=len(probs), graph=graph)
     else:
         raise SystemExit(f"unknown workload {name!r} (rosenbrock, dijet, mlp, mixed)")
     call()  # warm-up
@@ -132,7 +134,40 @@ def run(name: str, calls: int, max_iter: int | None, gradient: str) -> dict:
         out["bin_evals_per_s"] = evals * 100 / best
     if name == "mlp":
         out["start_iterations_per_s"] = out["sum_iter"] / best
+    if name == "mixed" and not graph:
+        out["classes"] = class_table(probs, names, res)
     return out
+
+
+def class_table(probs, names, res) -> dict:
+    """Per class (objective and dimension) of a mixed batch: its problems and
+    starts, its PSO and BFGS device times, and its work rates.  A class's
+    problems share one PSO and one BFGS launch sequence; side by side with
+    the other classes once the shapes are settled.  For the Gaussian
+    classes on the tile forms the BFGS is FP64-MFMA bound: its algorithmic
+    rate is 2 D^2 flops per objective evaluation (first point and every
+    line-search trial; the gradient is the product's by-product) over the
+    class's BFGS time, against the 78.6 TFLOP/s FP64 peak.  The other classes
+    are bound by their objective's VALU work (transcendentals) or latency,
+    and report evaluations per second."""
+    by = {}
+    for pr, name, r in zip(probs, names, res):
+        c = by.setdefault(name, dict(problems=0, starts=0, converged=0, evals=0, iters=0,
+                                     ms_pso=r.ms_pso, ms_bfgs=r.ms_opt, dim=pr.f.dim))
+        c["problems"] += 1
+        c["starts"] += pr.N
+        c["converged"] += r.c.claimed
+        c["evals"] += r.sum_ls_evals + pr.N
+        c["iters"] += r.sum_iter
+    for name, c in by.items():
+        t = c["ms_bfgs"] * 1e-3
+        c["evals_per_s"] = c["evals"] / t if t > 0 else None
+        c["start_iterations_per_s"] = c["iters"] / t if t > 0 else None
+        if name.startswith("gaussian") and t > 0:
+            tf = 2.0 * c["dim"] ** 2 * c["evals"] / t / 1e12
+            c["roofline"] = {"bound": "mfma", "achieved_tflops": tf, "peak": 78.6,
+                             "frac": tf / 78.6}
+    return by
 
 
 def main(argv: list[str]) -> int:
@@ -141,8 +176,10 @@ def main(argv: list[str]) -> int:
     ap.add_argument("--calls", type=int, default=2)
     ap.add_argument("--max-iter", type=int, default=None)
     ap.add_argument("--gradient", default="adjoint", choices=["adjoint", "dual"])
+    ap.add_argument("--graph", action="store_true",
+                    help="mixed: replay the batch from a captured hipGraph once settled")
     a = ap.parse_args(argv)
-    print(json.dumps(run(a.workload, a.calls, a.max_iter, a.gradient)), flush=True)
+    print(json.dumps(run(a.workload, a.calls, a.max_iter, a.gradient, a.graph)), flush=True)
     return 0
 
 
