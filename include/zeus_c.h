@@ -93,7 +93,7 @@ typedef int32_t zeus_status_t;
                               evaluating its next two line-search trials
                               per round; same per-start arithmetic as
                               ZEUS_FORM_TILE (Gaussian, dim >= 16, adjoint
-                              gradient); part of ZEUS_FORM_GROUP and AUTO  */
+                              gradient); only when asked for              */
 
 /* Trajectory recording of one problem (reference zeus.cuh:62-103,189-239;
    recording in bfgs_sequential.cuh:100-109 and bfgs_parallel.cuh:155-184).

@@ -72,11 +72,12 @@ pair_unit_range(const double* Cf, const double* xs, const double* ps, double a, 
                 int lk)
 {
   using S = CsShape<D>;
-  const double* frag = Cf + (int64_t(mb) * S::KB << 6) + (li + 16 * lk);
+  const gdouble_ptr frag = as_global(Cf) + (int64_t(mb) * S::KB << 6) + (li + 16 * lk);
   double av[KB1 - KB0];
 #pragma unroll
   for (int kb = KB0; kb < KB1; ++kb)
     av[kb - KB0] = frag[kb << 6];
+  __builtin_amdgcn_sched_barrier(0); // every fragment load issued before the chain
   mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int kb = KB0; kb < KB1; ++kb) {

@@ -33,16 +33,30 @@ struct CsShape {
   static constexpr int XS = kCsTile + 1;     // padded LDS row stride of Xt / G
 };
 
+// A device-memory pointer as a global-address-space pointer: loads through
+// it are global loads, counted by vmcnt alone.  (Through a generic pointer
+// they are flat loads, which also count in lgkmcnt, so every wait for an
+// LDS read would wait for the fragment loads in flight as well.)
+typedef const double __attribute__((address_space(1)))* gdouble_ptr;
+ZEUS_D ZEUS_FI gdouble_ptr
+as_global(const double* p)
+{
+  return reinterpret_cast<gdouble_ptr>(reinterpret_cast<uint64_t>(p));
+}
+
 template <int D, int KB0, int KB1>
 ZEUS_D ZEUS_FI mfma_acc_t
 cs_unit_range(const double* Cf, const double* xt, int mb, int li, int lk)
 {
   using S = CsShape<D>;
-  const double* frag = Cf + (int64_t(mb) * S::KB << 6) + (li + 16 * lk);
+  const gdouble_ptr frag = as_global(Cf) + (int64_t(mb) * S::KB << 6) + (li + 16 * lk);
   double av[KB1 - KB0];
 #pragma unroll
   for (int kb = KB0; kb < KB1; ++kb)
     av[kb - KB0] = frag[kb << 6];
+  // every fragment load is issued before the first MFMA waits on one (the
+  // scheduler would otherwise interleave them with a shallow window)
+  __builtin_amdgcn_sched_barrier(0);
   mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int kb = KB0; kb < KB1; ++kb)
@@ -63,7 +77,8 @@ struct CsFrag {
   ZEUS_D ZEUS_FI void load(const double* Cf, int u, int li, int lk)
   {
     const int kb0 = (u & 1) ? S::KB1 : 0, n = (u & 1) ? S::KB - S::KB1 : S::KB1;
-    const double* frag = Cf + (int64_t(u >> 1) * S::KB << 6) + (li + 16 * lk) + (kb0 << 6);
+    const gdouble_ptr frag =
+      as_global(Cf) + (int64_t(u >> 1) * S::KB << 6) + (li + 16 * lk) + (kb0 << 6);
 #pragma unroll
     for (int k = 0; k < N; ++k)
       av[k] = k < n ? frag[k << 6] : 0.0;

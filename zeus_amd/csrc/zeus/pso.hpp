@@ -127,7 +127,17 @@ pso_fold_read(const PsoArgs& a, int pi, const ProblemDesc& P, int t_prev)
   const int q = t_prev & 1;
   const int nblk = int((P.n_starts + PPB - 1) / PPB);
   FoldIn f{__builtin_nan(""), int64_t(1) << 62, __builtin_huge_val(), 0.0};
-  for (int b = threadIdx.x; b < nblk; b += NT) {
+  // this thread's first partial replaces the initial (NaN, 2^62) in any case
+  // (NaN loses to a number; two NaNs go to the lower index), so it is only
+  // loaded here and compared in pso_fold_finish: nothing waits for these
+  // loads before the caller issues its other loads
+  int b = threadIdx.x;
+  if (b < nblk) {
+    const int64_t o = int64_t(q) * a.n_blocks + P.block0 + b;
+    f.bv = a.part_val[o];
+    f.bi = a.part_idx[o];
+  }
+  for (b += NT; b < nblk; b += NT) {
     const int64_t o = int64_t(q) * a.n_blocks + P.block0 + b;
     const double v = a.part_val[o];
     const int64_t i = a.part_idx[o];

@@ -325,10 +325,12 @@ candidates(const ClassOps* ops, const zeus_options& opt)
     opt.form == ZEUS_FORM_GROUP || opt.form == ZEUS_FORM_AUTO || !ops->thread_ok;
   if (want_thread && ops->thread_ok)
     v.push_back({ZEUS_FORM_THREAD, 0});
-  if (want_group && tile_ok && opt.group_threads <= 0) {
-    v.push_back({ZEUS_FORM_TILE_PAIRED, dev::kTileThreads});
+  // the paired tile form is chosen explicitly only: it spends two products
+  // per round where the single form spends one, and the product phase is
+  // what bounds a round at these shapes (measured slower for the 150-D
+  // headline, with and without early stop)
+  if (want_group && tile_ok && opt.group_threads <= 0)
     v.push_back({ZEUS_FORM_TILE, dev::kTileThreads});
-  }
   if ((want_group || opt.form == ZEUS_FORM_TEAM) && opt.group_threads <= 0)
     for (int k = 0; k < ops->n_team; ++k)
       v.push_back({ZEUS_FORM_TEAM, ops->team_ts[k]});
