@@ -87,13 +87,6 @@ typedef int32_t zeus_status_t;
                               start, 64 / team starts per wavefront
                               (Rosenbrock, Rastrigin, Ackley, sphere with
                               small dim); part of ZEUS_FORM_GROUP and AUTO  */
-#define ZEUS_FORM_TILE_PAIRED 5 /* the tile form as two half-tiles of 8
-                              starts running out of phase (one evaluates
-                              on MFMA while the other decides), every start
-                              evaluating its next two line-search trials
-                              per round; same per-start arithmetic as
-                              ZEUS_FORM_TILE (Gaussian, dim >= 16, adjoint
-                              gradient); only when asked for              */
 
 /* Trajectory recording of one problem (reference zeus.cuh:62-103,189-239;
    recording in bfgs_sequential.cuh:100-109 and bfgs_parallel.cuh:155-184).

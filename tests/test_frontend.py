@@ -117,7 +117,7 @@ def test_frontend_builtin_gaussian150(orc, gpu_cases):
     D = 150
     C = np.full((D, D), 0.5)
     np.fill_diagonal(C, 1.0 + (D - 1) * 0.5)
-    assert gpu_cases["gaussian150_builtin_form"]["form"] in (3, 5)  # ZEUS_FORM_TILE{,_PAIRED}
+    assert gpu_cases["gaussian150_builtin_form"]["form"] == 3  # ZEUS_FORM_TILE
     got = gpu_cases["gaussian150_builtin"]
     want = orc.run(orc.GAUSSIAN, D, -5, 5, 64, 10000, 10, 65, 1e-8, 42, params=C.reshape(-1),
                    early_stop=False)

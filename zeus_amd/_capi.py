@@ -17,7 +17,7 @@ UNSET, SURRENDERED, CONVERGED, STOPPED, MALLOC, KERNEL, NONFINITE = -1, 0, 1, 2,
 OBJ_ROSENBROCK, OBJ_RASTRIGIN, OBJ_ACKLEY, OBJ_GAUSSIAN, OBJ_LOGLIK, OBJ_MLP, OBJ_SPHERE = (
     1, 2, 3, 4, 5, 6, 7)
 GRAD_ADJOINT, GRAD_DUAL = 0, 1
-FORM_AUTO, FORM_THREAD, FORM_GROUP, FORM_TILE, FORM_TEAM, FORM_TILE_PAIRED = 0, 1, 2, 3, 4, 5
+FORM_AUTO, FORM_THREAD, FORM_GROUP, FORM_TILE, FORM_TEAM = 0, 1, 2, 3, 4
 FLAG_METRICS = 1
 PRNG_XORWOW, PRNG_PHILOX, PRNG_SOBOL = 0, 1, 2
 

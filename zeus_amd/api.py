@@ -258,7 +258,7 @@ def _options(gradient="adjoint", form=None, parallel=True, group_threads=0, nzer
     elif isinstance(form, str):
         form = {"auto": capi.FORM_AUTO, "thread": capi.FORM_THREAD,
                 "group": capi.FORM_GROUP, "tile": capi.FORM_TILE,
-                "team": capi.FORM_TEAM, "tile_paired": capi.FORM_TILE_PAIRED}[form]
+                "team": capi.FORM_TEAM}[form]
     return capi.Options(g, int(form), int(group_threads), int(nzerosteps),
                         1 if use_graph else 0, capi.FLAG_METRICS if metrics else 0)
 

@@ -230,37 +230,31 @@ struct Ops {
     return 0;
   }
 
-  // paired = 0: single tile form (bfgs_tile.hpp); 1: paired (bfgs_tile_paired.hpp)
-  static hipError_t bfgs_tile(const dev::TileArgs& a, int grid, int paired, hipStream_t s)
+  // tile form (bfgs_tile.hpp)
+  static hipError_t bfgs_tile(const dev::TileArgs& a, int grid, hipStream_t s)
   {
     if constexpr (!kTileOk) {
       return hipErrorInvalidConfiguration;
     } else {
       if (fault_take_launch())
         return hipErrorLaunchFailure;
-      if (paired)
-        dev::bfgs_tile_paired_kernel<D><<<grid, dev::kTileThreads, 0, s>>>(a);
-      else
-        dev::bfgs_tile_kernel<D><<<grid, dev::kTileThreads, 0, s>>>(a);
+      dev::bfgs_tile_kernel<D><<<grid, dev::kTileThreads, 0, s>>>(a);
       return hipGetLastError();
     }
   }
-  static int tile_blocks_per_cu(int paired)
+  static int tile_blocks_per_cu()
   {
     if constexpr (!kTileOk) {
       return 1;
     } else {
-      static int cached[2] = {0, 0};
-      int& c = cached[paired ? 1 : 0];
-      if (c == 0) {
+      static int cached = 0;
+      if (cached == 0) {
         int n = 0;
-        const void* k = paired ? reinterpret_cast<const void*>(&dev::bfgs_tile_paired_kernel<D>)
-                               : reinterpret_cast<const void*>(&dev::bfgs_tile_kernel<D>);
-        const hipError_t e =
-          hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, dev::kTileThreads, 0);
-        c = (e == hipSuccess && n > 0) ? n : 1;
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &n, reinterpret_cast<const void*>(&dev::bfgs_tile_kernel<D>), dev::kTileThreads, 0);
+        cached = (e == hipSuccess && n > 0) ? n : 1;
       }
-      return c;
+      return cached;
     }
   }
 

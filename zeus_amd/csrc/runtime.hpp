@@ -6,7 +6,6 @@
 #include "zeus/bfgs_team.hpp"
 #include "zeus/bfgs_thread.hpp"
 #include "zeus/bfgs_tile.hpp"
-#include "zeus/bfgs_tile_paired.hpp"
 #include "zeus/prng.hpp"
 #include "zeus/pso.hpp"
 #include "zeus/summary.hpp"
@@ -37,11 +36,10 @@ struct ClassOps {
   hipError_t (*bfgs_team)(const dev::ClassArgs&, int gradient, int ts, int grid,
                           hipStream_t) = nullptr;
   int (*team_blocks_per_cu)(int gradient, int ts) = nullptr;
-  // tile forms (16 starts per workgroup, objective on MFMA; adjoint only):
-  // paired = 0 single, 1 paired (two half-tiles, two trials per round)
+  // tile form (16 starts per workgroup, objective on MFMA; adjoint only)
   bool tile_ok = false;
-  hipError_t (*bfgs_tile)(const dev::TileArgs&, int grid, int paired, hipStream_t) = nullptr;
-  int (*tile_blocks_per_cu)(int paired) = nullptr;
+  hipError_t (*bfgs_tile)(const dev::TileArgs&, int grid, hipStream_t) = nullptr;
+  int (*tile_blocks_per_cu)() = nullptr;
   int64_t tile_slot_doubles = 0; // doubles of global memory per start slot
   hipError_t (*eval)(const dev::ProblemDesc&, const double* x, int64_t n, double* f,
                      double* g, int gradient, hipStream_t) = nullptr;

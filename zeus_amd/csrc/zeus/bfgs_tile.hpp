@@ -68,18 +68,54 @@ struct TileSlot {
   double best, alpha, ddir;
 };
 
+// The product's segments: 16 (one per wavefront), the first kTileKeep
+// fragments of each held in registers across rounds.
+#ifndef ZEUS_TILE_KEEP
+#define ZEUS_TILE_KEEP 64
+#endif
+constexpr int kTileKeep = ZEUS_TILE_KEEP;
+#ifndef ZEUS_HG_CHUNK
+#define ZEUS_HG_CHUNK 2
+#endif
+constexpr int kHgChunk = ZEUS_HG_CHUNK; // rank-2 pairs read together in H g
+template <int D>
+using TileSeg = CsFullSeg<D, kTileWaves, kTileKeep>;
+
 template <int D>
 struct TileLds {
   using S = TileShape<D>;
+  using Seg = TileSeg<D>;
   double xt[S::KP * S::XS]; // submitted points, [row][slot]
-  double gt[2 * D * S::XS]; // Cs Xt by k half, [half][row][slot]
-  double fpart[S::U * kTileStarts];
+  double gpart[Seg::NP * Seg::kSlot]; // partial blocks of Cs Xt, [part][row % 16][slot]
+  double fpart[Seg::NP * kTileStarts];
+  int cover[S::MB * Seg::kCover]; // parts of each row block, in segment order
   double x[kTileStarts * D], g[kTileStarts * D], p[kTileStarts * D];
   int live[kTileStarts];
+  int newconv[kTileStarts]; // slot converged in the last round, not yet counted
   int stop; // early stop seen at the start of this round
   TileSlot slot[kTileStarts];
   ProblemDesc prob; // the tile's problem
 };
+
+// Row d of G for slot j: the partial blocks of the segments covering d's row
+// block, added in segment order.
+template <int D>
+ZEUS_D ZEUS_FI double
+tile_grad(const TileLds<D>& L, int d, int j)
+{
+  using Seg = TileSeg<D>;
+  using S = TileShape<D>;
+  const int* cv = L.cover + (d >> 4) * Seg::kCover;
+  const int o = (d & 15) * S::XS + j;
+  double g = L.gpart[cv[0] * Seg::kSlot + o];
+#pragma unroll
+  for (int c = 1; c < Seg::kCover; ++c) {
+    const int p = cv[c];
+    if (p >= 0)
+      g += L.gpart[p * Seg::kSlot + o];
+  }
+  return g;
+}
 
 template <int EL>
 ZEUS_D ZEUS_FI double
@@ -96,13 +132,16 @@ tile_dot(const double (&a)[EL], const double (&b)[EL])
 // value at the slot's submitted point.
 template <int D>
 ZEUS_D void
-tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
+tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
                const StartOut& out, double* mem, double f)
 {
   using S = TileShape<D>;
   constexpr int EL = S::EL;
-  TileSlot& st = L.slot[w];
-  const int lane = threadIdx.x % kWave;
+  // the lane index is made opaque here, so that the per-lane addresses below
+  // are recomputed each round instead of being held (or spilled) across the
+  // whole round loop
+  int lane = threadIdx.x % kWave;
+  asm volatile("" : "+v"(lane));
   double* X = L.x + w * D;
   double* G = L.g + w * D;
   double* Pd = L.p + w * D;
@@ -128,7 +167,7 @@ tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
   auto load_grad = [&](double (&r)[EL]) { // G column: k halves in fixed order
 #pragma unroll
     for (int e = 0; e < EL; ++e)
-      r[e] = ok[e] ? L.gt[row[e] * S::XS + w] + L.gt[(D + row[e]) * S::XS + w] : 0.0;
+      r[e] = ok[e] ? tile_grad<D>(L, row[e], w) : 0.0;
   };
   auto submit = [&](int phase) {
     double x[EL], p[EL];
@@ -232,7 +271,7 @@ tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
       v[e] = 0.0;
 #pragma unroll 1
     for (int c = 0; c < D; ++c) {
-      const double gc = L.gt[c * S::XS + w] + L.gt[(D + c) * S::XS + w];
+      const double gc = tile_grad<D>(L, c, w);
 #pragma unroll
       for (int e = 0; e < EL; ++e)
         if (ok[e])
@@ -243,17 +282,63 @@ tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
     for (int e = 0; e < EL; ++e)
       v[e] = gn[e];
   }
-#pragma unroll 2
-  for (int j = 0; j < st.k; ++j) { // 2 pairs in flight; v accumulates in order j
-    double sj[EL], aj[EL];
-    load(hist + int64_t(j) * 2 * D, sj);
-    load(hist + int64_t(j) * 2 * D + D, aj);
-    const double cj = tile_dot<EL>(aj, gn);
-    const double dj = tile_dot<EL>(sj, gn);
+  // pairs in chunks of kHgChunk: every load of a chunk is issued before its
+  // dot products (one memory latency per chunk; the pairs of the whole chip
+  // are read in the same rounds, mostly beyond the L2), the chunk's 2
+  // kHgChunk wavefront sums run side by side, then v accumulates in order j
+  constexpr int kC = kHgChunk;
+  // no LDS access is moved across the pair loop (the values it would keep
+  // live through the loop cost registers the chunk needs)
+  asm volatile("" ::: "memory");
+#pragma unroll 1
+  for (int j0 = 0; j0 < st.k; j0 += kC) {
+    double sj[kC][EL], aj[kC][EL];
 #pragma unroll
-    for (int e = 0; e < EL; ++e)
-      v[e] = ::fma(sj[e], cj, ::fma(aj[e], dj, v[e]));
+    for (int c = 0; c < kC; ++c) {
+      // past the list's end the last pair is loaded again; its terms are not
+      // used.  Rows past D read the next vector of the slot (in bounds: the
+      // explicit B follows the list) and are zeroed after the load, so the
+      // loads are unconditional, addressed by a uniform base and the lane.
+      const int j = j0 + c < st.k ? j0 + c : st.k - 1;
+      const gdouble_ptr pj = as_global(hist) + int64_t(j) * 2 * D + lane;
+#pragma unroll
+      for (int e = 0; e < EL; ++e) {
+        sj[c][e] = pj[e * kWave];
+        aj[c][e] = pj[D + e * kWave];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kC; ++c)
+#pragma unroll
+      for (int e = 0; e < EL; ++e) {
+        sj[c][e] = ok[e] ? sj[c][e] : 0.0;
+        aj[c][e] = ok[e] ? aj[c][e] : 0.0;
+      }
+    double cj[kC], dj[kC];
+#pragma unroll
+    for (int c = 0; c < kC; ++c) {
+      double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+      for (int e = 0; e < EL; ++e) {
+        t0 = ::fma(aj[c][e], gn[e], t0);
+        t1 = ::fma(sj[c][e], gn[e], t1);
+      }
+      cj[c] = t0;
+      dj[c] = t1;
+    }
+#pragma unroll
+    for (int c = 0; c < kC; ++c) {
+      cj[c] = wave_sum(cj[c]);
+      dj[c] = wave_sum(dj[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < kC; ++c)
+      if (j0 + c < st.k)
+#pragma unroll
+        for (int e = 0; e < EL; ++e)
+          v[e] = ::fma(sj[c][e], cj[c], ::fma(aj[c][e], dj[c], v[e]));
   }
+  asm volatile("" ::: "memory");
   // s = xt - x, y = gn - g, u = H y = H gn - H g = v + p
   double s[EL];
   double sTy, yu;
@@ -357,13 +442,33 @@ tile_slot_step(TileLds<D>& L, const int w, const ProblemDesc& P, Control* ctl,
   }
   if (nrm < P.tol) {
     if (lane == 0)
-      (void)__hip_atomic_fetch_add(&ctl->converged, 1, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT); // no wait for the old value
+      L.newconv[w] = 1; // counted for the tile at the top of the next round
     finish(kStatusConverged, st.best, nrm);
     return;
   }
   ++st.iter;
   begin_iteration();
+}
+
+constexpr int kStopReader = kTileThreads - kWave; // thread that counts and reads
+
+// Adds the slots flagged in newconv to the problem's converged count (one
+// relaxed atomic, none when no slot converged) and clears the flags; with
+// `read`, returns the count after the addition.
+template <int D>
+ZEUS_D ZEUS_FI int
+tile_count_converged(TileLds<D>& L, Control* ctl, bool read)
+{
+  int n = 0;
+#pragma unroll
+  for (int k = 0; k < kTileStarts; ++k) {
+    n += L.newconv[k];
+    L.newconv[k] = 0;
+  }
+  if (n > 0)
+    (void)__hip_atomic_fetch_add(&ctl->converged, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return read ? __hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+              : 0;
 }
 
 template <int D>
@@ -374,13 +479,17 @@ bfgs_tile_kernel(TileArgs a)
   __shared__ TileLds<D> L;
   const int tid = threadIdx.x, lane = tid % kWave;
   const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
-  const int li = lane & 15, lk = lane >> 4;
+  const int li0 = lane & 15, lk0 = lane >> 4;
   auto slot_mem = [&](int j) {
     return a.slot_mem + (int64_t(blockIdx.x) * kTileStarts + j) * S::kSlotDoubles;
   };
 
+  using Seg = TileSeg<D>;
   for (int e = tid; e < S::KP * S::XS; e += kTileThreads)
     L.xt[e] = 0.0; // padding rows stay zero
+  if (tid < S::MB * Seg::kCover)
+    L.cover[tid] = Seg::cover(tid / Seg::kCover, tid % Seg::kCover);
+  Seg seg;
 
   for (int64_t tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
     const int pi = wave_uniform(a.tile_problem[tile]);
@@ -395,6 +504,7 @@ bfgs_tile_kernel(TileArgs a)
     for (int j = w; j < kTileStarts; j += kTileWaves) {
       TileSlot& st = L.slot[j];
       st = TileSlot{-1, 0, 0, 0, 0, 0, 0, 0, 0.0, 1.0, 0.0};
+      L.newconv[j] = 0;
       const int64_t sl = (tile - P.block0) * kTileStarts + j; // local start
       if (sl < P.n_starts) {
         st.s = P.start0 + sl;
@@ -410,6 +520,8 @@ bfgs_tile_kernel(TileArgs a)
       }
     }
     Control* ctl = a.ctl + pi;
+    if constexpr (Seg::kKeep > 0) // this problem's fragments kept across rounds
+      seg.load(w, Cf, li0, lk0, 0, Seg::kKeep);
     __syncthreads();
 
     for (;;) {
@@ -418,40 +530,33 @@ bfgs_tile_kernel(TileArgs a)
       // last wavefront (one product unit) before its unit and used after it,
       // so the load's latency hides behind the MFMA work; a start sees it at
       // the top of its next iteration, as in the other forms.
-      constexpr int kStopReader = kTileThreads - kWave;
+      // per-lane indices made opaque each round: addresses derived from them
+      // are recomputed here rather than held across the round loop
+      int li = li0, lk = lk0;
+      asm volatile("" : "+v"(li), "+v"(lk));
+      // The starts that converged in the last round are added to the
+      // problem's count here, one atomic per tile and round (one per start,
+      // all on one address, queued up behind each other in the rounds where
+      // most starts of the chip converge together).
       int conv = 0;
       if (tid == kStopReader)
-        conv = __hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      // ---- G = Cs Xt by (row block, k half) units and partial values ----
-      for (int u = w; u < S::U; u += kTileWaves) {
-        const int mb = u >> 1, kh = u & 1;
-        const mfma_acc_t acc = cs_unit<D>(Cf, L.xt, u, li, lk);
-        double* gp = L.gt + kh * D * S::XS;
-        double fp = 0.0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int rr = 16 * mb + 4 * q + lk;
-          if (rr < D) {
-            gp[rr * S::XS + li] = acc[q];
-            fp = ::fma(L.xt[rr * S::XS + li], acc[q], fp);
-          }
-        }
-        fp += __shfl_xor(fp, 16, kWave);
-        fp += __shfl_xor(fp, 32, kWave);
-        if (lk == 0)
-          L.fpart[u * kTileStarts + li] = fp;
-      }
+        conv = tile_count_converged(L, ctl, true);
+      // ---- G = Cs Xt in balanced segments, partial blocks and values ----
+      if constexpr (Seg::kKeep < Seg::LMAX)
+        seg.load(w, Cf, li, lk, Seg::kKeep, Seg::LMAX);
+      __builtin_amdgcn_sched_barrier(0);
+      seg.mma(w, L.xt, L.gpart, L.fpart, li, lk);
       if (tid == kStopReader)
         L.stop = P.required > 0 && conv >= P.required;
       __syncthreads();
       // ---- start logic: wavefront w runs its slots one after the other ----
       for (int j = w; j < kTileStarts; j += kTileWaves) {
         if (L.slot[j].phase != 0) {
-          // f = 0.5 sum_u fpart[u]: lane u holds unit u's partial, one
+          // f = 0.5 sum_p fpart[p]: lane p holds part p's partial, one
           // fixed-order wavefront sum
-          static_assert(S::U <= kWave, "one unit per lane");
-          const double fu = lane < S::U ? L.fpart[lane * kTileStarts + j] : 0.0;
-          tile_slot_step<D>(L, j, P, ctl, a.out, slot_mem(j), 0.5 * wave_sum(fu));
+          static_assert(Seg::NP <= kWave, "one part per lane");
+          const double fu = lane < Seg::NP ? L.fpart[lane * kTileStarts + j] : 0.0;
+          tile_slot_step<D>(L, L.slot[j], j, P, a.out, slot_mem(j), 0.5 * wave_sum(fu));
         }
         if (lane == 0)
           L.live[j] = L.slot[j].phase != 0;
@@ -464,6 +569,8 @@ bfgs_tile_kernel(TileArgs a)
       if (!any)
         break;
     }
+    if (tid == kStopReader)
+      (void)tile_count_converged(L, ctl, false);
     __syncthreads(); // LDS reuse by the next tile
   }
 }

@@ -65,6 +65,60 @@ cs_unit_range(const double* Cf, const double* xt, int mb, int li, int lk)
   return acc;
 }
 
+// Two units of the same k range (row blocks mb0 and mb1) run by one
+// wavefront: the fragment loads of both are issued before either chain
+// starts (alternating, in the order the chains consume them), and each Xt
+// operand is read once for both.  Each accumulator sums its unit in the
+// order cs_unit_range does, so the results are those of two cs_unit_range
+// calls.
+template <int D, int KB0, int KB1>
+ZEUS_D ZEUS_FI void
+cs_unit_pair_range(const double* Cf, const double* xt, int mb0, int mb1, int li, int lk,
+                   mfma_acc_t& acc0, mfma_acc_t& acc1)
+{
+  using S = CsShape<D>;
+  const gdouble_ptr f0 = as_global(Cf) + (int64_t(mb0) * S::KB << 6) + (li + 16 * lk);
+  const gdouble_ptr f1 = as_global(Cf) + (int64_t(mb1) * S::KB << 6) + (li + 16 * lk);
+  double a0[KB1 - KB0], a1[KB1 - KB0];
+#pragma unroll
+  for (int kb = KB0; kb < KB1; ++kb) {
+    a0[kb - KB0] = f0[kb << 6];
+    a1[kb - KB0] = f1[kb << 6];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  acc0 = mfma_acc_t{0.0, 0.0, 0.0, 0.0};
+  acc1 = mfma_acc_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kb = KB0; kb < KB1; ++kb) {
+    const double b = xt[(4 * kb + lk) * S::XS + li];
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[kb - KB0], b, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[kb - KB0], b, acc1, 0, 0, 0);
+  }
+  // Xt operands are read kAhead steps ahead of their MFMAs, not all at once
+  // (both units' fragments already hold 4 (KB1 - KB0) registers)
+  constexpr int N = KB1 - KB0, kAhead = N < 3 ? N : 3;
+  __builtin_amdgcn_sched_group_barrier(0x0100, kAhead, 0); // LDS reads
+#pragma unroll
+  for (int i = 0; i < N - kAhead; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x0008, 2, 0);    // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);
+  }
+  __builtin_amdgcn_sched_group_barrier(0x0008, 2 * kAhead, 0);
+}
+
+// Units u0 and u0 + 2 m (same k half) as one pair; see cs_unit_pair_range.
+template <int D>
+ZEUS_D ZEUS_FI void
+cs_unit_pair(const double* Cf, const double* xt, int u0, int u1, int li, int lk,
+             mfma_acc_t& acc0, mfma_acc_t& acc1)
+{
+  using S = CsShape<D>;
+  if (u0 & 1)
+    cs_unit_pair_range<D, S::KB1, S::KB>(Cf, xt, u0 >> 1, u1 >> 1, li, lk, acc0, acc1);
+  else
+    cs_unit_pair_range<D, 0, S::KB1>(Cf, xt, u0 >> 1, u1 >> 1, li, lk, acc0, acc1);
+}
+
 // The same product split in two steps, so that a caller can issue a unit's
 // fragment loads early (before the points are staged) and run the MFMA
 // chain later: CsFrag<D>::load, then CsFrag<D>::mma.
@@ -191,21 +245,21 @@ struct CsUpperSeg {
   double av[LMAX];
 
   // Fragments of segment w (wave-uniform): one code path for every
-  // wavefront, the segment's start a scalar; elements past the segment's
-  // length are set to zero (never used).  Global loads with immediate
-  // offsets (the buffer is device memory).
+  // wavefront, the segment's start a scalar.  Global loads (the buffer is
+  // device memory).
   ZEUS_D ZEUS_FI void load(int w, const double* Cu, int li, int lk)
   {
+    w = __builtin_amdgcn_readfirstlane(w);
     if (w >= NA)
       return;
     const int lo = int(int64_t(U::T) * w / NA), len = int(int64_t(U::T) * (w + 1) / NA) - lo;
-    typedef const double __attribute__((address_space(1)))* gptr;
-    const gptr frag =
-      reinterpret_cast<gptr>(reinterpret_cast<uint64_t>(Cu + (int64_t(lo) << 6)));
-    const int lane_off = li + 16 * lk;
+    const gdouble_ptr frag = as_global(Cu) + (int64_t(lo) << 6) + (li + 16 * lk);
+    // unconditional loads (a branch per load would split them into basic
+    // blocks): past the segment's end the last fragment is loaded again, and
+    // never used
 #pragma unroll
     for (int i = 0; i < LMAX; ++i)
-      av[i] = i < len ? frag[(i << 6) + lane_off] : 0.0;
+      av[i] = frag[(i < len ? i : len - 1) << 6];
   }
 
   template <int W>
@@ -255,6 +309,167 @@ struct CsUpperSeg {
         mma_w<W>(xt, fslot, li, lk);
       else
         mma<W + 1>(w, xt, fslot, li, lk);
+    }
+  }
+};
+
+// ---- the full product in balanced segments ----------------------------------
+//
+// The MB x KB steps (row block by row block, k ascending: the fragment order
+// of the buffer) cut into NA contiguous segments of near-equal length,
+// segment w run by wavefront w.  A segment covers parts of one or two row
+// blocks; each part leaves its 16 x 16 partial block of G in an LDS slot
+// (16 rows, stride XS) and its partial value x . G in a value slot.  Row
+// d of G is the sum of the slots of the segments covering its row block, in
+// segment order (cover(): at most kCover of them); f is the sum of the value
+// slots in slot order.  At D = 150: 380 steps, 23-24 per wavefront, 24 parts.
+// A segment's first kKeep fragments can be held in registers across rounds
+// (Cs does not change), so a round's chain starts on them while the rest
+// arrive from L2.
+namespace full_detail {
+template <int D>
+constexpr int steps()
+{
+  return CsShape<D>::MB * CsShape<D>::KB;
+}
+template <int D, int NA>
+constexpr int lo(int w)
+{
+  return int(int64_t(steps<D>()) * w / NA);
+}
+template <int D>
+constexpr int mb_of(int m)
+{
+  return m / CsShape<D>::KB;
+}
+template <int D, int NA>
+constexpr int parts_in(int w)
+{
+  return lo<D, NA>(w + 1) > lo<D, NA>(w)
+           ? mb_of<D>(lo<D, NA>(w + 1) - 1) - mb_of<D>(lo<D, NA>(w)) + 1
+           : 0;
+}
+template <int D, int NA>
+constexpr int part_base(int w)
+{
+  int p = 0;
+  for (int v = 0; v < w; ++v)
+    p += parts_in<D, NA>(v);
+  return p;
+}
+// segments covering row block mb: first and last (inclusive)
+template <int D, int NA>
+constexpr int seg_first(int mb)
+{
+  int w = 0;
+  while (lo<D, NA>(w + 1) <= mb * CsShape<D>::KB)
+    ++w;
+  return w;
+}
+template <int D, int NA>
+constexpr int seg_last(int mb)
+{
+  int w = 0;
+  while (lo<D, NA>(w + 1) <= (mb + 1) * CsShape<D>::KB - 1)
+    ++w;
+  return w;
+}
+template <int D, int NA>
+constexpr int max_cover()
+{
+  int c = 0;
+  for (int mb = 0; mb < CsShape<D>::MB; ++mb) {
+    const int n = seg_last<D, NA>(mb) - seg_first<D, NA>(mb) + 1;
+    c = n > c ? n : c;
+  }
+  return c;
+}
+} // namespace full_detail
+
+template <int D, int NA, int KEEP = 0>
+struct CsFullSeg {
+  using S = CsShape<D>;
+  static constexpr int T = full_detail::steps<D>();
+  static constexpr int LMAX = (T + NA - 1) / NA;
+  static constexpr int kKeep = KEEP < LMAX ? KEEP : LMAX;
+  static constexpr int NP = full_detail::part_base<D, NA>(NA); // partial blocks
+  static constexpr int kSlot = 16 * S::XS;                     // doubles per block
+  static constexpr int kCover = full_detail::max_cover<D, NA>();
+  double av[LMAX];
+
+  // Part index of cover entry c of row block mb (-1: none).  Plain integer
+  // code, for filling a table once.
+  static ZEUS_HD constexpr int cover(int mb, int c)
+  {
+    const int w = full_detail::seg_first<D, NA>(mb) + c;
+    if (w > full_detail::seg_last<D, NA>(mb))
+      return -1;
+    return full_detail::part_base<D, NA>(w) + mb -
+           full_detail::mb_of<D>(full_detail::lo<D, NA>(w));
+  }
+
+  // Fragments i0 .. i1 - 1 of segment w (wave-uniform); elements past the
+  // segment's length are zero and never used.
+  ZEUS_D ZEUS_FI void load(int w, const double* Cf, int li, int lk, int i0 = 0, int i1 = LMAX)
+  {
+    w = __builtin_amdgcn_readfirstlane(w);
+    if (w >= NA)
+      return;
+    const int lo = int(int64_t(T) * w / NA), len = int(int64_t(T) * (w + 1) / NA) - lo;
+    const gdouble_ptr frag = as_global(Cf) + (int64_t(lo) << 6) + (li + 16 * lk);
+    // unconditional loads (a branch per load would split them into basic
+    // blocks, each opened by a wait): past the segment's end the last
+    // fragment is loaded again, and never used
+#pragma unroll
+    for (int i = 0; i < LMAX; ++i)
+      if (i >= i0 && i < i1)
+        av[i] = frag[(i < len ? i : len - 1) << 6];
+  }
+
+  template <int W>
+  ZEUS_D ZEUS_FI void mma_w(const double* xt, double* gpart, double* fpart, int li, int lk) const
+  {
+    constexpr int LO = full_detail::lo<D, NA>(W), LEN = full_detail::lo<D, NA>(W + 1) - LO;
+    constexpr int MB0 = full_detail::mb_of<D>(LO);
+    constexpr int PB = full_detail::part_base<D, NA>(W);
+    mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
+    [&]<int... I>(std::integer_sequence<int, I...>) {
+      (
+        [&] {
+          constexpr int m = LO + I, mb = full_detail::mb_of<D>(m), kb = m - mb * S::KB;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[I], xt[(4 * kb + lk) * S::XS + li], acc,
+                                                     0, 0, 0);
+          if constexpr (I == LEN - 1 || kb == S::KB - 1) {
+            constexpr int part = PB + mb - MB0;
+            double* gp = gpart + part * kSlot;
+            double fp = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int r = 4 * q + lk, rr = 16 * mb + r;
+              gp[r * S::XS + li] = acc[q];
+              if (rr < D)
+                fp = ::fma(xt[rr * S::XS + li], acc[q], fp);
+            }
+            fp += __shfl_xor(fp, 16, kWave);
+            fp += __shfl_xor(fp, 32, kWave);
+            if (lk == 0)
+              fpart[part * kCsTile + li] = fp;
+            acc = mfma_acc_t{0.0, 0.0, 0.0, 0.0};
+          }
+        }(),
+        ...);
+    }(std::make_integer_sequence<int, LEN>{});
+  }
+
+  template <int W = 0>
+  ZEUS_D ZEUS_FI void mma(int w, const double* xt, double* gpart, double* fpart, int li,
+                          int lk) const
+  {
+    if constexpr (W < NA) {
+      if (w == W)
+        mma_w<W>(xt, gpart, fpart, li, lk);
+      else
+        mma<W + 1>(w, xt, gpart, fpart, li, lk);
     }
   }
 };

@@ -318,17 +318,11 @@ candidates(const ClassOps* ops, const zeus_options& opt)
   const bool tile_ok = ops->tile_ok && opt.gradient == ZEUS_GRAD_ADJOINT;
   if (opt.form == ZEUS_FORM_TILE && tile_ok)
     return {{ZEUS_FORM_TILE, dev::kTileThreads}};
-  if (opt.form == ZEUS_FORM_TILE_PAIRED && tile_ok)
-    return {{ZEUS_FORM_TILE_PAIRED, dev::kTileThreads}};
   const bool want_thread = opt.form == ZEUS_FORM_THREAD || opt.form == ZEUS_FORM_AUTO;
   const bool want_group =
     opt.form == ZEUS_FORM_GROUP || opt.form == ZEUS_FORM_AUTO || !ops->thread_ok;
   if (want_thread && ops->thread_ok)
     v.push_back({ZEUS_FORM_THREAD, 0});
-  // the paired tile form is chosen explicitly only: it spends two products
-  // per round where the single form spends one, and the product phase is
-  // what bounds a round at these shapes (measured slower for the 150-D
-  // headline, with and without early stop)
   if (want_group && tile_ok && opt.group_threads <= 0)
     v.push_back({ZEUS_FORM_TILE, dev::kTileThreads});
   if ((want_group || opt.form == ZEUS_FORM_TEAM) && opt.group_threads <= 0)
@@ -485,8 +479,8 @@ launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st, int t
 int
 group_grid(const ClassPlan& c, const TuneVal& v, int gradient)
 {
-  if (v.form == ZEUS_FORM_TILE || v.form == ZEUS_FORM_TILE_PAIRED) {
-    const int occ = c.ops->tile_blocks_per_cu(v.form == ZEUS_FORM_TILE_PAIRED);
+  if (v.form == ZEUS_FORM_TILE) {
+    const int occ = c.ops->tile_blocks_per_cu();
     const int g = int(std::min<int64_t>(c.pso_blocks, int64_t(g_ds.n_cu) * occ));
     return std::max(g, 1);
   }
@@ -506,7 +500,7 @@ slab_doubles(const ClassPlan& c, const TuneVal& v, int gradient)
 {
   if (v.form == ZEUS_FORM_THREAD)
     return c.ops->thread_slab_per_start * c.total;
-  if (v.form == ZEUS_FORM_TILE || v.form == ZEUS_FORM_TILE_PAIRED)
+  if (v.form == ZEUS_FORM_TILE)
     return c.ops->tile_slot_doubles * dev::kTileStarts * group_grid(c, v, gradient);
   if (v.form == ZEUS_FORM_TEAM)
     return 0; // H in registers
@@ -522,11 +516,10 @@ launch_bfgs_cfg(ClassPlan& c, const TuneVal& v, Control* ctl_all, int gradient, 
     return hipSuccess;
   if (v.form == ZEUS_FORM_THREAD)
     return c.ops->bfgs_thread(a, gradient, st);
-  if (v.form == ZEUS_FORM_TILE || v.form == ZEUS_FORM_TILE_PAIRED) {
+  if (v.form == ZEUS_FORM_TILE) {
     const dev::TileArgs ta{c.d_desc, c.d_block_problem, c.pso_blocks, ctl_all + c.ctl0, c.out,
                            c.h_slab};
-    return c.ops->bfgs_tile(ta, group_grid(c, v, gradient), v.form == ZEUS_FORM_TILE_PAIRED,
-                            st);
+    return c.ops->bfgs_tile(ta, group_grid(c, v, gradient), st);
   }
   if (v.form == ZEUS_FORM_TEAM)
     return c.ops->bfgs_team(a, gradient, v.threads, group_grid(c, v, gradient), st);
@@ -672,13 +665,11 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     if (c.traced) {
       // recording runs one shape, not the tile or team form, and is not
       // timed against others (each trial would rewrite the trajectory)
-      if (opt.form == ZEUS_FORM_TILE || opt.form == ZEUS_FORM_TILE_PAIRED ||
-          opt.form == ZEUS_FORM_TEAM)
+      if (opt.form == ZEUS_FORM_TILE || opt.form == ZEUS_FORM_TEAM)
         return ZEUS_E_ARG;
       std::vector<TuneVal> keep;
       for (const auto& v : c.cands)
-        if (v.form != ZEUS_FORM_TILE && v.form != ZEUS_FORM_TILE_PAIRED &&
-            v.form != ZEUS_FORM_TEAM)
+        if (v.form != ZEUS_FORM_TILE && v.form != ZEUS_FORM_TEAM)
           keep.push_back(v);
       if (keep.empty())
         keep.push_back({ZEUS_FORM_GROUP, c.ops->group_nt[c.ops->n_group - 1]});
