@@ -6,11 +6,12 @@
 // first point, a line-search trial x + alpha p, or the unconditional point
 // after 20 rejected trials).  The 16 points Xt (D x 16, LDS) are evaluated
 // together: G = Cs Xt on FP64 MFMA (v_mfma_f64_16x16x4f64), Cs = (C + C^T)/2
-// streamed from L2 as A operands, f_j = 0.5 Xt_j . G_j, and G_j is the
-// adjoint gradient of start j at its point.  The product is split into
-// (16-row block, half of the k range) units spread evenly over the 16
-// wavefronts (and so over the four SIMDs); the two k halves land in two LDS
-// partial buffers that are added in a fixed order.  Wavefront w then runs
+// as A operands, f_j = 0.5 Xt_j . G_j, and G_j is the adjoint gradient of
+// start j at its point.  The product's MB x KB steps are cut into 16
+// balanced segments, one per wavefront (cs_product.hpp, CsFullSeg), whose
+// Cs fragments stay in registers for the whole tile (Cs does not change);
+// each segment leaves partial blocks of G in LDS, and row d of G is the sum
+// of the blocks of its row block in segment order.  Wavefront w then runs
 // start w's BFGS logic for the round -- line-search decision, update,
 // convergence test -- with DPP wavefront reductions only, and submits its
 // next point.  Two workgroup barriers per round.
@@ -68,16 +69,11 @@ struct TileSlot {
   double best, alpha, ddir;
 };
 
-// The product's segments: 16 (one per wavefront), the first kTileKeep
-// fragments of each held in registers across rounds.
-#ifndef ZEUS_TILE_KEEP
-#define ZEUS_TILE_KEEP 64
-#endif
-constexpr int kTileKeep = ZEUS_TILE_KEEP;
-#ifndef ZEUS_HG_CHUNK
-#define ZEUS_HG_CHUNK 2
-#endif
-constexpr int kHgChunk = ZEUS_HG_CHUNK; // rank-2 pairs read together in H g
+// The product's segments: 16, one per wavefront, every fragment of a
+// segment held in registers across the rounds of a tile (kTileKeep >= the
+// longest segment; 24 fragments = 48 VGPRs per lane at D = 150).
+constexpr int kTileKeep = 64;
+constexpr int kHgChunk = 2; // rank-2 pairs read together in H g
 template <int D>
 using TileSeg = CsFullSeg<D, kTileWaves, kTileKeep>;
 

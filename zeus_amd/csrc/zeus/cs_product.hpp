@@ -7,10 +7,9 @@
 // as columns in LDS, row-major with a padded stride of 17 doubles (rows D ..
 // KP-1 zero).  v_mfma_f64_16x16x4f64 operand layout: A[i][k] on lane
 // i + 16k, B[k][j] on lane j + 16k, result D[i][j] in register i % 4 of lane
-// j + 16 (i / 4).  The product is cut into units (16-row block of Cs, half
-// of the k range); a unit loads all its Cs fragments (one coalesced 512-byte
-// read each) before its MFMA chain starts, so the L2 latency is paid once
-// per unit.
+// j + 16 (i / 4).  Both products below cut their MFMA steps into contiguous
+// segments, one per wavefront, whose fragments (one coalesced 512-byte read
+// each) are loaded before the segment's MFMA chain starts.
 #pragma once
 
 #include <utility>
@@ -28,8 +27,6 @@ struct CsShape {
   static constexpr int MB = (D + 15) / 16;   // 16-row blocks of Cs
   static constexpr int KB = (D + 3) / 4;     // 4-deep k steps
   static constexpr int KP = KB * 4;          // padded depth (rows of Xt)
-  static constexpr int KB1 = (KB + 1) / 2;   // k steps of the first half
-  static constexpr int U = 2 * MB;           // (row block, k half) units
   static constexpr int XS = kCsTile + 1;     // padded LDS row stride of Xt / G
 };
 
@@ -42,125 +39,6 @@ ZEUS_D ZEUS_FI gdouble_ptr
 as_global(const double* p)
 {
   return reinterpret_cast<gdouble_ptr>(reinterpret_cast<uint64_t>(p));
-}
-
-template <int D, int KB0, int KB1>
-ZEUS_D ZEUS_FI mfma_acc_t
-cs_unit_range(const double* Cf, const double* xt, int mb, int li, int lk)
-{
-  using S = CsShape<D>;
-  const gdouble_ptr frag = as_global(Cf) + (int64_t(mb) * S::KB << 6) + (li + 16 * lk);
-  double av[KB1 - KB0];
-#pragma unroll
-  for (int kb = KB0; kb < KB1; ++kb)
-    av[kb - KB0] = frag[kb << 6];
-  // every fragment load is issued before the first MFMA waits on one (the
-  // scheduler would otherwise interleave them with a shallow window)
-  __builtin_amdgcn_sched_barrier(0);
-  mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int kb = KB0; kb < KB1; ++kb)
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kb - KB0], xt[(4 * kb + lk) * S::XS + li], acc,
-                                               0, 0, 0);
-  return acc;
-}
-
-// Two units of the same k range (row blocks mb0 and mb1) run by one
-// wavefront: the fragment loads of both are issued before either chain
-// starts (alternating, in the order the chains consume them), and each Xt
-// operand is read once for both.  Each accumulator sums its unit in the
-// order cs_unit_range does, so the results are those of two cs_unit_range
-// calls.
-template <int D, int KB0, int KB1>
-ZEUS_D ZEUS_FI void
-cs_unit_pair_range(const double* Cf, const double* xt, int mb0, int mb1, int li, int lk,
-                   mfma_acc_t& acc0, mfma_acc_t& acc1)
-{
-  using S = CsShape<D>;
-  const gdouble_ptr f0 = as_global(Cf) + (int64_t(mb0) * S::KB << 6) + (li + 16 * lk);
-  const gdouble_ptr f1 = as_global(Cf) + (int64_t(mb1) * S::KB << 6) + (li + 16 * lk);
-  double a0[KB1 - KB0], a1[KB1 - KB0];
-#pragma unroll
-  for (int kb = KB0; kb < KB1; ++kb) {
-    a0[kb - KB0] = f0[kb << 6];
-    a1[kb - KB0] = f1[kb << 6];
-  }
-  __builtin_amdgcn_sched_barrier(0);
-  acc0 = mfma_acc_t{0.0, 0.0, 0.0, 0.0};
-  acc1 = mfma_acc_t{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int kb = KB0; kb < KB1; ++kb) {
-    const double b = xt[(4 * kb + lk) * S::XS + li];
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[kb - KB0], b, acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[kb - KB0], b, acc1, 0, 0, 0);
-  }
-  // Xt operands are read kAhead steps ahead of their MFMAs, not all at once
-  // (both units' fragments already hold 4 (KB1 - KB0) registers)
-  constexpr int N = KB1 - KB0, kAhead = N < 3 ? N : 3;
-  __builtin_amdgcn_sched_group_barrier(0x0100, kAhead, 0); // LDS reads
-#pragma unroll
-  for (int i = 0; i < N - kAhead; ++i) {
-    __builtin_amdgcn_sched_group_barrier(0x0008, 2, 0);    // MFMA
-    __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);
-  }
-  __builtin_amdgcn_sched_group_barrier(0x0008, 2 * kAhead, 0);
-}
-
-// Units u0 and u0 + 2 m (same k half) as one pair; see cs_unit_pair_range.
-template <int D>
-ZEUS_D ZEUS_FI void
-cs_unit_pair(const double* Cf, const double* xt, int u0, int u1, int li, int lk,
-             mfma_acc_t& acc0, mfma_acc_t& acc1)
-{
-  using S = CsShape<D>;
-  if (u0 & 1)
-    cs_unit_pair_range<D, S::KB1, S::KB>(Cf, xt, u0 >> 1, u1 >> 1, li, lk, acc0, acc1);
-  else
-    cs_unit_pair_range<D, 0, S::KB1>(Cf, xt, u0 >> 1, u1 >> 1, li, lk, acc0, acc1);
-}
-
-// The same product split in two steps, so that a caller can issue a unit's
-// fragment loads early (before the points are staged) and run the MFMA
-// chain later: CsFrag<D>::load, then CsFrag<D>::mma.
-template <int D>
-struct CsFrag {
-  using S = CsShape<D>;
-  static constexpr int N = S::KB - S::KB1 > S::KB1 ? S::KB - S::KB1 : S::KB1;
-  double av[N];
-
-  ZEUS_D ZEUS_FI void load(const double* Cf, int u, int li, int lk)
-  {
-    const int kb0 = (u & 1) ? S::KB1 : 0, n = (u & 1) ? S::KB - S::KB1 : S::KB1;
-    const gdouble_ptr frag =
-      as_global(Cf) + (int64_t(u >> 1) * S::KB << 6) + (li + 16 * lk) + (kb0 << 6);
-#pragma unroll
-    for (int k = 0; k < N; ++k)
-      av[k] = k < n ? frag[k << 6] : 0.0;
-  }
-
-  ZEUS_D ZEUS_FI mfma_acc_t mma(const double* xt, int u, int li, int lk) const
-  {
-    const int kb0 = (u & 1) ? S::KB1 : 0, n = (u & 1) ? S::KB - S::KB1 : S::KB1;
-    mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < N; ++k)
-      if (k < n)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], xt[(4 * (kb0 + k) + lk) * S::XS + li],
-                                                   acc, 0, 0, 0);
-    return acc;
-  }
-};
-
-// Unit u: rows 16 (u / 2) .. + 15 of Cs, k half u % 2.  Lane (li, lk) of the
-// result holds G[16 (u / 2) + 4 q + lk][li] in component q (partial over
-// the unit's k half).
-template <int D>
-ZEUS_D ZEUS_FI mfma_acc_t
-cs_unit(const double* Cf, const double* xt, int u, int li, int lk)
-{
-  using S = CsShape<D>;
-  return (u & 1) ? cs_unit_range<D, S::KB1, S::KB>(Cf, xt, u >> 1, li, lk)
-                 : cs_unit_range<D, 0, S::KB1>(Cf, xt, u >> 1, li, lk);
 }
 
 // ---- values only: the upper block triangle ----------------------------------
