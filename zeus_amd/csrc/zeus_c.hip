@@ -83,6 +83,9 @@ struct TuneVal {
   int form, threads;
 };
 
+// timing events (see ensure_device's note on the system-scope fence)
+constexpr unsigned kTimingEventFlags = hipEventDisableSystemFence;
+
 struct DeviceState {
   bool init = false;
   int dev = 0;
@@ -157,16 +160,23 @@ ensure_device()
 
   if (hipStreamCreateWithFlags(&g_ds.stream, hipStreamNonBlocking) != hipSuccess)
     return ZEUS_E_LAUNCH;
+  // Events order work on this device and time it; none is used to make
+  // memory visible to the host (the results come back through a stream-
+  // ordered copy and a stream synchronisation), so they are recorded without
+  // the system-scope fence, whose cache writeback and invalidation otherwise
+  // idles the GPU for several microseconds at every record between kernels.
   for (auto& e : g_ds.ev)
-    if (hipEventCreate(&e) != hipSuccess)
+    if (hipEventCreateWithFlags(&e, kTimingEventFlags) != hipSuccess)
       return ZEUS_E_LAUNCH;
   for (auto& sd : g_ds.side)
     if (hipStreamCreateWithFlags(&sd, hipStreamNonBlocking) != hipSuccess)
       return ZEUS_E_LAUNCH;
-  if (hipEventCreateWithFlags(&g_ds.fork, hipEventDisableTiming) != hipSuccess)
+  if (hipEventCreateWithFlags(&g_ds.fork, hipEventDisableTiming | hipEventDisableSystemFence) !=
+      hipSuccess)
     return ZEUS_E_LAUNCH;
   for (auto& e : g_ds.join)
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence) !=
+        hipSuccess)
       return ZEUS_E_LAUNCH;
   g_ds.init = true;
   return ZEUS_OK;
@@ -794,7 +804,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
   // per class: PSO start, PSO end, BFGS end (classes are timed one by one)
   while (g_ds.cls_ev.size() < 3 * classes.size()) {
     hipEvent_t ev;
-    if (hipEventCreate(&ev) != hipSuccess)
+    if (hipEventCreateWithFlags(&ev, kTimingEventFlags) != hipSuccess)
       return fail(hipErrorOutOfMemory);
     g_ds.cls_ev.push_back(ev);
   }
