@@ -808,6 +808,16 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       return fail(hipErrorOutOfMemory);
     g_ds.cls_ev.push_back(ev);
   }
+  // One class one after the other: its own events also time the whole call
+  // (each event record between kernels costs the GPU a few microseconds).
+  bool any_traj = false;
+  for (const auto& c : classes)
+    for (const auto& d : c.desc)
+      any_traj = any_traj || d.traj.fval != nullptr;
+  const bool one = classes.size() == 1;
+  const hipEvent_t ev_begin = one ? g_ds.cls_ev[0] : g_ds.ev[0];
+  const hipEvent_t ev_pso = one && !any_traj ? g_ds.cls_ev[1] : g_ds.ev[1];
+  const hipEvent_t ev_end = one ? g_ds.cls_ev[2] : g_ds.ev[2];
   auto sequence = [&](bool timed) -> hipError_t {
     hipError_t err = hipSuccess;
     // the controls and the first class's gBest coordinates in one launch
@@ -871,7 +881,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       }
       return hipGetLastError();
     }
-    if (timed)
+    if (timed && ev_begin == g_ds.ev[0])
       (void)hipEventRecord(g_ds.ev[0], st);
     for (size_t k = 0; k < classes.size(); ++k) {
       if (timed)
@@ -891,7 +901,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
         }
     if ((err = hipGetLastError()) != hipSuccess)
       return err;
-    if (timed)
+    if (timed && ev_pso == g_ds.ev[1])
       (void)hipEventRecord(g_ds.ev[1], st);
     for (size_t k = 0; k < classes.size(); ++k) {
       auto& c = classes[k];
@@ -901,7 +911,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
         (void)hipEventRecord(g_ds.cls_ev[3 * k + 2], st);
       g_ds.last[{c.ops->objective, c.ops->dim, opt.gradient}] = c.cfg;
     }
-    if (timed)
+    if (timed && ev_end == g_ds.ev[2])
       (void)hipEventRecord(g_ds.ev[2], st);
     for (auto& c : classes) {
       dev::summary_kernel<<<int(c.probs.size()), dev::kSummaryThreads, 0, st>>>(
@@ -979,7 +989,7 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
       // side by side, a class's BFGS follows its own PSO; one after the other,
       // it follows the previous class's BFGS (or the end of every PSO phase)
       const hipEvent_t b0 = concurrent ? g_ds.cls_ev[3 * k + 1]
-                            : k == 0   ? g_ds.ev[1]
+                            : k == 0   ? ev_pso
                                        : g_ds.cls_ev[3 * (k - 1) + 2];
       (void)hipEventElapsedTime(&cls_pso[k], g_ds.cls_ev[3 * k], g_ds.cls_ev[3 * k + 1]);
       (void)hipEventElapsedTime(&cls_opt[k], b0, g_ds.cls_ev[3 * k + 2]);
@@ -990,8 +1000,8 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
         ms_opt = std::max(ms_opt, cls_opt[k]);
       }
     } else {
-      (void)hipEventElapsedTime(&ms_pso, g_ds.ev[0], g_ds.ev[1]);
-      (void)hipEventElapsedTime(&ms_opt, g_ds.ev[1], g_ds.ev[2]);
+      (void)hipEventElapsedTime(&ms_pso, ev_begin, ev_pso);
+      (void)hipEventElapsedTime(&ms_opt, ev_pso, ev_end);
     }
   }
 
