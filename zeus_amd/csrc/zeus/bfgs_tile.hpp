@@ -449,11 +449,10 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
 constexpr int kStopReader = kTileThreads - kWave; // thread that counts and reads
 
 // Adds the slots flagged in newconv to the problem's converged count (one
-// relaxed atomic, none when no slot converged) and clears the flags; with
-// `read`, returns the count after the addition.
+// relaxed atomic, none when no slot converged) and clears the flags.
 template <int D>
-ZEUS_D ZEUS_FI int
-tile_count_converged(TileLds<D>& L, Control* ctl, bool read)
+ZEUS_D ZEUS_FI void
+tile_count_converged(TileLds<D>& L, Control* ctl)
 {
   int n = 0;
 #pragma unroll
@@ -463,8 +462,6 @@ tile_count_converged(TileLds<D>& L, Control* ctl, bool read)
   }
   if (n > 0)
     (void)__hip_atomic_fetch_add(&ctl->converged, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return read ? __hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-              : 0;
 }
 
 template <int D>
@@ -533,15 +530,22 @@ bfgs_tile_kernel(TileArgs a)
       // The starts that converged in the last round are added to the
       // problem's count here, one atomic per tile and round (one per start,
       // all on one address, queued up behind each other in the rounds where
-      // most starts of the chip converge together).
-      int conv = 0;
+      // most starts of the chip converge together).  The count is read a
+      // third of the way through the product, when the other tiles' sums of
+      // the last round have been added, and its latency hides behind the
+      // rest of the product.
       if (tid == kStopReader)
-        conv = tile_count_converged(L, ctl, true);
+        tile_count_converged(L, ctl);
+      int conv = 0;
+      auto read_count = [&]() {
+        if (tid == kStopReader)
+          conv = __hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      };
       // ---- G = Cs Xt in balanced segments, partial blocks and values ----
       if constexpr (Seg::kKeep < Seg::LMAX)
         seg.load(w, Cf, li, lk, Seg::kKeep, Seg::LMAX);
       __builtin_amdgcn_sched_barrier(0);
-      seg.mma(w, L.xt, L.gpart, L.fpart, li, lk);
+      seg.mma(w, L.xt, L.gpart, L.fpart, li, lk, read_count);
       if (tid == kStopReader)
         L.stop = P.required > 0 && conv >= P.required;
       __syncthreads();
@@ -566,7 +570,7 @@ bfgs_tile_kernel(TileArgs a)
         break;
     }
     if (tid == kStopReader)
-      (void)tile_count_converged(L, ctl, false);
+      tile_count_converged(L, ctl);
     __syncthreads(); // LDS reuse by the next tile
   }
 }

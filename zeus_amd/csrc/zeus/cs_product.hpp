@@ -304,8 +304,11 @@ struct CsFullSeg {
         av[i] = frag[(i < len ? i : len - 1) << 6];
   }
 
-  template <int W>
-  ZEUS_D ZEUS_FI void mma_w(const double* xt, double* gpart, double* fpart, int li, int lk) const
+  // `hook` runs once, a third of the way through the segment's MFMA chain
+  // (a load issued there completes behind the rest of the chain)
+  template <int W, class Hook>
+  ZEUS_D ZEUS_FI void mma_w(const double* xt, double* gpart, double* fpart, int li, int lk,
+                            const Hook& hook) const
   {
     constexpr int LO = full_detail::lo<D, NA>(W), LEN = full_detail::lo<D, NA>(W + 1) - LO;
     constexpr int MB0 = full_detail::mb_of<D>(LO);
@@ -315,6 +318,11 @@ struct CsFullSeg {
       (
         [&] {
           constexpr int m = LO + I, mb = full_detail::mb_of<D>(m), kb = m - mb * S::KB;
+          if constexpr (I == LEN / 3) {
+            __builtin_amdgcn_sched_barrier(0);
+            hook();
+            __builtin_amdgcn_sched_barrier(0);
+          }
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[I], xt[(4 * kb + lk) * S::XS + li], acc,
                                                      0, 0, 0);
           if constexpr (I == LEN - 1 || kb == S::KB - 1) {
@@ -339,15 +347,15 @@ struct CsFullSeg {
     }(std::make_integer_sequence<int, LEN>{});
   }
 
-  template <int W = 0>
-  ZEUS_D ZEUS_FI void mma(int w, const double* xt, double* gpart, double* fpart, int li,
-                          int lk) const
+  template <int W = 0, class Hook>
+  ZEUS_D ZEUS_FI void mma(int w, const double* xt, double* gpart, double* fpart, int li, int lk,
+                          const Hook& hook) const
   {
     if constexpr (W < NA) {
       if (w == W)
-        mma_w<W>(xt, gpart, fpart, li, lk);
+        mma_w<W>(xt, gpart, fpart, li, lk, hook);
       else
-        mma<W + 1>(w, xt, gpart, fpart, li, lk);
+        mma<W + 1>(w, xt, gpart, fpart, li, lk, hook);
     }
   }
 };
