@@ -313,6 +313,8 @@ struct CsFullSeg {
     constexpr int LO = full_detail::lo<D, NA>(W), LEN = full_detail::lo<D, NA>(W + 1) - LO;
     constexpr int MB0 = full_detail::mb_of<D>(LO);
     constexpr int PB = full_detail::part_base<D, NA>(W);
+    if constexpr (LEN == 0) // fewer steps than segments: the hook still runs
+      hook();
     mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
     [&]<int... I>(std::integer_sequence<int, I...>) {
       (
