@@ -335,11 +335,13 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
           v[e] = ::fma(sj[c][e], cj[c], ::fma(aj[c][e], dj[c], v[e]));
   }
   asm volatile("" ::: "memory");
-  // s = xt - x, y = gn - g, u = H y = H gn - H g = v + p
+  // s = xt - x, y = gn - g, u = H y = H gn - H g = v + p; and |g'|^2 for
+  // the gradient g' after the iteration (gn if accepted, else g), summed as
+  // the norm of g' would be once it is stored
   double s[EL];
-  double sTy, yu;
+  double sTy, yu, gg;
   {
-    double t0 = 0.0, t1 = 0.0;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
 #pragma unroll
     for (int e = 0; e < EL; ++e) {
       const double xv = ok[e] ? X[row[e]] : 0.0;
@@ -350,10 +352,14 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
       const double y = gn[e] - gv;
       t0 = ::fma(s[e], y, t0);
       t1 = ::fma(y, v[e] + pv, t1);
+      const double ga = acc ? gn[e] : gv;
+      t2 = ::fma(ga, ga, t2);
     }
     sTy = wave_sum(t0);
     yu = wave_sum(t1);
+    gg = wave_sum(t2);
   }
+  double dd = 0.0; // g'.p' (the next iteration's directional derivative), in row order
   const bool upd = !(::fabs(sTy) < kCurvatureSkip);
   if (upd) {
     const double rho = 1.0 / sTy;
@@ -376,11 +382,16 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
     }
     // p' = -H' g = -(H g + s (a.g) + a (s.g)), H g = v (accepted) or -p
 #pragma unroll
-    for (int e = 0; e < EL; ++e)
+    for (int e = 0; e < EL; ++e) {
+      const double ga = acc ? gn[e] : (ok[e] ? G[row[e]] : 0.0);
+      double pn = 0.0;
       if (ok[e]) {
         const double hg = acc ? v[e] : -Pd[row[e]];
-        Pd[row[e]] = -::fma(a[e], q1, ::fma(s[e], q0, hg));
+        pn = -::fma(a[e], q1, ::fma(s[e], q0, hg));
+        Pd[row[e]] = pn;
       }
+      dd = ::fma(ga, pn, dd);
+    }
     // append the pair; a full list is folded into B first
     if (st.k == S::kHist) {
 #pragma unroll 1
@@ -418,9 +429,16 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
     ++st.k;
   } else if (acc) {
 #pragma unroll
-    for (int e = 0; e < EL; ++e)
+    for (int e = 0; e < EL; ++e) {
+      const double pn = ok[e] ? -v[e] : 0.0;
       if (ok[e])
-        Pd[row[e]] = -v[e];
+        Pd[row[e]] = pn;
+      dd = ::fma(gn[e], pn, dd);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      dd = ::fma(ok[e] ? G[row[e]] : 0.0, ok[e] ? Pd[row[e]] : 0.0, dd);
   }
   if (acc) {
     st.best = fnew;
@@ -431,7 +449,8 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
         G[row[e]] = gn[e];
       }
   }
-  const double nrm = gnorm_now();
+  const double nrm = ::sqrt(gg);
+  const double ddir = wave_sum(dd);
   if (!::isfinite(nrm) || !::isfinite(fnew)) {
     finish(kStatusNonFinite, fnew, nrm);
     return;
@@ -442,8 +461,21 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
     finish(kStatusConverged, st.best, nrm);
     return;
   }
+  // top of the next iteration (begin_iteration with the norm and the
+  // directional derivative already at hand)
   ++st.iter;
-  begin_iteration();
+  if (st.iter >= P.max_iter) {
+    finish(kStatusSurrendered, st.best, nrm);
+    return;
+  }
+  if (L.stop != 0) {
+    finish(kStatusStopped, st.best, nrm);
+    return;
+  }
+  st.ddir = ddir;
+  st.alpha = 1.0;
+  st.trial = 0;
+  submit(2);
 }
 
 constexpr int kStopReader = kTileThreads - kWave; // thread that counts and reads
