@@ -211,3 +211,26 @@ def test_tile_repeatable_without_early_stop(prng, N):
         assert np.array_equal(a.per_start[key], b.per_start[key]), key
     assert a.idx == b.idx and a.fval == b.fval and a.sum_ls_evals == b.sum_ls_evals
     assert a.c.claimed == int((a.per_start["status"] == 1).sum()) == N
+
+
+def test_tile_more_tiles_than_workgroups():
+    """4500 starts make 282 tiles: with one workgroup per CU some workgroups
+    run two tiles one after the other (slots, counts and the stop flag are
+    reset per tile).  Without early stop two calls agree bit for bit and
+    every start converges to the minimum at 0; with required = 100 the
+    converged count (added per tile and round) equals the starts with status
+    1, and every start is converged or stopped."""
+    C = gauss_matrix(150)
+    f = z.Gaussian(C)
+    N = 4500
+    a, b = (z.Zeus(f, -5, 5, N, 10000, 10, N + 1, "gaussian", 1e-8, 42, 0, prng=z.PRNGType.PHILOX,
+                   form="tile", per_start=True) for _ in range(2))
+    for key in ("status", "iter", "fval", "x"):
+        assert np.array_equal(a.per_start[key], b.per_start[key]), key
+    assert set(a.per_start["status"]) == {1} and a.c.claimed == N
+    assert np.abs(a.per_start["x"]).max() <= 1e-6
+    r = z.Zeus(f, -5, 5, N, 10000, 10, 100, "gaussian", 1e-8, 42, 0, prng=z.PRNGType.PHILOX,
+               form="tile", per_start=True)
+    st = r.per_start["status"]
+    assert r.c.claimed >= 100 and r.c.claimed == int((st == 1).sum())
+    assert set(st) <= {1, 2} and r.c.claimed + r.c.stopped == N
