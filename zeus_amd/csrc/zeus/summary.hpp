@@ -61,13 +61,10 @@ traj_fill_kernel(TrajOut t, int D)
 // C is read through the problem descriptor in device memory (uploaded by
 // every call), never taken as a kernel argument, so that a replayed graph
 // prepares the matrix of the call that replays it.
-static __global__ void
-symmetrize_kernel(const ProblemDesc* desc, int k, double* Cs, double* Cf, int D)
+ZEUS_D ZEUS_FI void
+symmetrize_element(const ProblemDesc* desc, int k, double* Cs, double* Cf, int D, int e)
 {
   const double* C = desc[k].params;
-  // element indices stay below D * D + (fragments) * 64 < 2^31: 32-bit
-  // division (64-bit division is a long software sequence)
-  const int e = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
   if (e < D * D) {
     const int i = e / D, j = e - i * D;
     Cs[e] = 0.5 * (C[int64_t(i) * D + j] + C[int64_t(j) * D + i]);
@@ -99,6 +96,32 @@ symmetrize_kernel(const ProblemDesc* desc, int k, double* Cs, double* Cf, int D)
       (r < D && c < D) ? 0.5 * (C[int64_t(r) * D + c] + C[int64_t(c) * D + r]) : 0.0;
     Cf[int64_t(MB) * KB * 64 + e] = kb < 4 * mb + 4 ? 0.5 * v : v;
   }
+}
+
+// element indices stay below D * D + (fragments) * 64 < 2^31: 32-bit
+// division (64-bit division is a long software sequence)
+static __global__ void
+symmetrize_kernel(const ProblemDesc* desc, int k, double* Cs, double* Cf, int D)
+{
+  symmetrize_element(desc, k, Cs, Cf, D, int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x));
+}
+
+// init_controls_kernel and symmetrize_kernel in one launch, for a call whose
+// first class is one Gaussian problem (the grid covers both element ranges).
+static __global__ void
+init_symmetrize_kernel(Control* c, int n, double* gbx, int64_t n_gbx, const ProblemDesc* desc,
+                       int k, double* Cs, double* Cf, int D)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    Control z{};
+    z.best_idx = -1;
+    z.gbest_val = __builtin_huge_val();
+    c[i] = z;
+  }
+  for (int64_t kk = i; kk < n_gbx; kk += int64_t(gridDim.x) * blockDim.x)
+    gbx[kk] = 0.0;
+  symmetrize_element(desc, k, Cs, Cf, D, int(i));
 }
 
 // Doubles of the Gaussian preparation per problem: Cs, its fragments, then

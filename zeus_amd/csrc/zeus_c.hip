@@ -384,6 +384,7 @@ struct ClassPlan {
   double* h_slab = nullptr;
   int ctl0 = 0;          // first control block of the class
   bool traced = false;   // some problem records a trajectory
+  bool prepared = false; // the objective was prepared by the call's first kernel
   bool metrics = false;  // per-start timing requested (ZEUS_FLAG_METRICS)
   bool xorwow = false;   // some problem runs PSO on XORWOW streams
   int keep_rng = 0;      // PSO phases: the caller keeps the XORWOW states
@@ -463,7 +464,7 @@ launch_class(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st, int t
   const int np = int(c.probs.size());
   Control* ctl = ctl_all + c.ctl0;
   hipError_t e = hipSuccess;
-  if (c.ops->objective == obj::kGaussian) {
+  if (c.ops->objective == obj::kGaussian && !c.prepared) {
     const int64_t na = dev::gaussian_aux_doubles(D);
     for (int k = 0; k < np; ++k) {
       double* cs = c.aux + k * na;
@@ -539,7 +540,12 @@ launch_bfgs_cfg(ClassPlan& c, const TuneVal& v, Control* ctl_all, int gradient, 
 
 // BFGS phase of a class.  With several candidate shapes (first call for this
 // key), each runs on the real inputs from the same PSO result and control
-// state; the fastest is cached and its results are the ones kept.
+// state; the fastest is cached and its results are the ones kept.  Each
+// candidate is timed twice and keeps its faster time, and the first one is
+// run once more before it is timed: a call made while the GPU's clocks are
+// still coming up (the first work of a process) would otherwise time its
+// first candidate slow and can keep a shape several times slower for every
+// later call.
 hipError_t
 launch_bfgs(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
 {
@@ -550,15 +556,21 @@ launch_bfgs(ClassPlan& c, Control* ctl_all, int gradient, hipStream_t st)
   size_t best = 0;
   hipError_t e = hipSuccess;
   for (size_t k = 0; k < c.cands.size(); ++k) {
-    dev::init_controls_kernel<<<(np + 255) / 256, 256, 0, st>>>(ctl_all + c.ctl0, np, nullptr, 0);
-    (void)hipEventRecord(g_ds.ev[4], st);
-    if ((e = launch_bfgs_cfg(c, c.cands[k], ctl_all, gradient, st)) != hipSuccess)
-      return e;
-    (void)hipEventRecord(g_ds.ev[5], st);
-    if ((e = hipEventSynchronize(g_ds.ev[5])) != hipSuccess)
-      return e;
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, g_ds.ev[4], g_ds.ev[5]);
+    for (int rep = k == 0 ? -1 : 0; rep < 2; ++rep) {
+      dev::init_controls_kernel<<<(np + 255) / 256, 256, 0, st>>>(ctl_all + c.ctl0, np, nullptr,
+                                                                  0);
+      (void)hipEventRecord(g_ds.ev[4], st);
+      if ((e = launch_bfgs_cfg(c, c.cands[k], ctl_all, gradient, st)) != hipSuccess)
+        return e;
+      (void)hipEventRecord(g_ds.ev[5], st);
+      if ((e = hipEventSynchronize(g_ds.ev[5])) != hipSuccess)
+        return e;
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, g_ds.ev[4], g_ds.ev[5]);
+      if (rep == 0 || (rep > 0 && t < ms))
+        ms = t;
+    }
     if (k == 0 || ms < best_ms) {
       best_ms = ms;
       best = k;
@@ -825,7 +837,16 @@ zeus_run_batch(const zeus_problem* probs, int32_t n_probs, const zeus_options* o
     for (size_t i = 0; i < classes.size(); ++i) {
       ClassPlan& c = classes[i];
       const int64_t ng = int64_t(c.probs.size()) * c.ops->dim;
-      if (i == 0)
+      // a first class of one Gaussian problem: its preparation (Cs and the
+      // fragments) joins this first launch
+      c.prepared = i == 0 && c.ops->objective == obj::kGaussian && c.probs.size() == 1;
+      if (c.prepared) {
+        const int D = c.ops->dim;
+        const int64_t na = dev::gaussian_aux_doubles(D);
+        const int64_t n = std::max<int64_t>(ctl_total, na);
+        dev::init_symmetrize_kernel<<<int((n + 255) / 256), 256, 0, st>>>(
+          d_ctl, ctl_total, c.gbx, ng, c.d_desc, 0, c.aux, c.aux + int64_t(D) * D, D);
+      } else if (i == 0)
         dev::init_controls_kernel<<<(ctl_total + 255) / 256, 256, 0, st>>>(d_ctl, ctl_total,
                                                                            c.gbx, ng);
       else
