@@ -41,7 +41,6 @@ constexpr int kTileStarts = kCsTile; // starts per tile = MFMA N dimension
 // tile (a 128-register budget per lane).
 constexpr int kTileThreads = 1024;
 constexpr int kTileWaves = kTileThreads / kWave;
-constexpr int kSlotsPerWave = kTileStarts / kTileWaves;
 static_assert(kTileStarts % kTileWaves == 0, "whole slots per wavefront");
 
 template <int D>
@@ -154,11 +153,6 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
 #pragma unroll
     for (int e = 0; e < EL; ++e)
       r[e] = ok[e] ? v[row[e]] : 0.0;
-  };
-  auto load_col = [&](const double* m, double (&r)[EL]) { // [row][slot] array
-#pragma unroll
-    for (int e = 0; e < EL; ++e)
-      r[e] = ok[e] ? m[row[e] * S::XS + w] : 0.0;
   };
   auto load_grad = [&](double (&r)[EL]) { // G column: k halves in fixed order
 #pragma unroll
