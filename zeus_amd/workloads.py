@@ -3,7 +3,7 @@ build one workload, run it warm, print one JSON line with its timing and
 counts.
 
     python -m zeus_amd.workloads rosenbrock|dijet|mlp|mixed [--calls K] [--max-iter M]
-                                 [--gradient G] [--graph]
+                                 [--gradient G] [--graph] [--form F [--group-threads T]]
 
 rosenbrock  Rosenbrock D = 2, box [-5, 5], 1024 starts, PSO 10, MAX_ITER 10000,
     required 100, tol 1e-8, seed 42 (the reference's CPU-runnable case).
@@ -22,7 +22,11 @@ mixed  64 problems in one batched call: Rosenbrock D in {2, 10, 50}, Rastrigin
     hipGraph (the per-class times are then not separated).
 
 The first call of a workload is a warm-up (code objects, pools, autotuner);
-the line reports the fastest of the timed calls.  Synthetic data, FP64.
+the line reports the fastest of the timed calls.  With --form (and
+--group-threads) the form is fixed, nothing is tuned, and the warm-up call
+runs 10 iterations: the way to time the full MAX_ITER 10000 of dijet and mlp,
+whose tuning call would otherwise run every shape to the cap.  Synthetic
+data, FP64.
 """
 from __future__ import annotations
 
@@ -85,38 +89,50 @@ def _mixed_problems(max_starts=None, max_iter=10000, early_stop=True):
     return probs, names
 
 
-def run(name: str, calls: int, max_iter: int | None, gradient: str, graph: bool = False) -> dict:
+def run(name: str, calls: int, max_iter: int | None, gradient: str, graph: bool = False,
+        form: str = "auto", group_threads: int = 0) -> dict:
     out = {"workload": name, "gradient": gradient, "data": "synthetic", "dtype": "f64"}
+    fixed = form != "auto"
+    kw = dict(form=form, group_threads=group_threads)
+    if fixed:
+        out.update(form=form, group_threads=group_threads)
     if name == "rosenbrock":
-        def call():
-            return [z.Zeus(z.Rosenbrock(2), -5, 5, 1024, max_iter or 10000, 10, 100,
-                           "rosenbrock", 1e-8, 42, 0, form="auto", gradient=gradient)]
+        it = max_iter or 10000
+
+        def make(n_it):
+            return lambda: [z.Zeus(z.Rosenbrock(2), -5, 5, 1024, n_it, 10, 100, "rosenbrock",
+                                   1e-8, 42, 0, gradient=gradient, **kw)]
         out["starts"] = 1024
     elif name == "dijet":
         it = max_iter or 100
-        probs = [z.Problem(z.LogLikelihood.from_params(z.make_dijet(42, e)), 0.0, 10.0, 256,
-                           it, 10, 100, 1e-8, 42, stream=e) for e in range(1024)]
 
-        def call():
-            return z.run_batch(probs, gradient=gradient)
+        def make(n_it):
+            probs = [z.Problem(z.LogLikelihood.from_params(z.make_dijet(42, e)), 0.0, 10.0, 256,
+                               n_it, 10, 100, 1e-8, 42, stream=e) for e in range(1024)]
+            return lambda: z.run_batch(probs, gradient=gradient, **kw)
         out.update(starts=1024 * 256, max_iter=it)
     elif name == "mlp":
         it = max_iter or 200
         d = z.make_mlp_data(42, 256)
         f = z.Mlp(d[:256], d[256:])
 
-        def call():
-            return [z.Zeus(f, -2, 2, 2048, it, 10, 100, "mlp", 1e-8, 42, 0, gradient=gradient)]
+        def make(n_it):
+            # without --form: Zeus's own default for parallel=True (the group form)
+            return lambda: [z.Zeus(f, -2, 2, 2048, n_it, 10, 100, "mlp", 1e-8, 42, 0,
+                                   gradient=gradient, **(kw if fixed else {}))]
         out.update(starts=2048, max_iter=it)
     elif name == "mixed":
-        probs, names = _mixed_problems()
+        it = max_iter or 10000
+        probs, names = _mixed_problems(max_iter=it)
 
-        def call():
-            return z.run_batch(probs, gradient=gradient, use_graph=graph)
+        def make(n_it):
+            return lambda: z.run_batch(probs, gradient=gradient, use_graph=graph, **kw)
         out.update(starts=sum(p.N for p in probs), problems=len(probs), graph=graph)
     else:
         raise SystemExit(f"unknown workload {name!r} (rosenbrock, dijet, mlp, mixed)")
-    call()  # warm-up
+    call = make(it)
+    # warm-up: with a fixed form only code objects and pools need it
+    (make(min(it, 10)) if fixed and name != "mixed" else call)()
     best, res = float("inf"), None
     for _ in range(calls):
         t = time.perf_counter()
@@ -178,8 +194,13 @@ def main(argv: list[str]) -> int:
     ap.add_argument("--gradient", default="adjoint", choices=["adjoint", "dual"])
     ap.add_argument("--graph", action="store_true",
                     help="mixed: replay the batch from a captured hipGraph once settled")
+    ap.add_argument("--form", default="auto", choices=["auto", "thread", "group", "team", "tile"],
+                    help="fix the BFGS form (no autotuning)")
+    ap.add_argument("--group-threads", type=int, default=0,
+                    help="workgroup size of the group form (with --form group)")
     a = ap.parse_args(argv)
-    print(json.dumps(run(a.workload, a.calls, a.max_iter, a.gradient, a.graph)), flush=True)
+    print(json.dumps(run(a.workload, a.calls, a.max_iter, a.gradient, a.graph, a.form,
+                         a.group_threads)), flush=True)
     return 0
 
 
