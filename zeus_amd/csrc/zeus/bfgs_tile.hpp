@@ -160,13 +160,20 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
       r[e] = ok[e] ? tile_grad<D>(L, row[e], w) : 0.0;
   };
   auto submit = [&](int phase) {
+    // every read before the first store (the compiler cannot tell the LDS
+    // rows apart); full rows need no mask (lane < 64)
+    const double al = st.alpha;
     double x[EL], p[EL];
-    load(X, x);
-    load(Pd, p);
 #pragma unroll
     for (int e = 0; e < EL; ++e)
-      if (ok[e])
-        L.xt[row[e] * S::XS + w] = ::fma(st.alpha, p[e], x[e]);
+      if ((e + 1) * kWave <= D || ok[e]) {
+        x[e] = X[row[e]];
+        p[e] = Pd[row[e]];
+      }
+#pragma unroll
+    for (int e = 0; e < EL; ++e)
+      if ((e + 1) * kWave <= D || ok[e])
+        L.xt[row[e] * S::XS + w] = ::fma(al, p[e], x[e]);
     st.phase = phase;
   };
   auto finish = [&](int status, double fval, double gnorm) {

@@ -22,11 +22,12 @@ mixed  64 problems in one batched call: Rosenbrock D in {2, 10, 50}, Rastrigin
     hipGraph (the per-class times are then not separated).
 
 The first call of a workload is a warm-up (code objects, pools, autotuner);
-the line reports the fastest of the timed calls.  With --form (and
---group-threads) the form is fixed, nothing is tuned, and the warm-up call
-runs 10 iterations: the way to time the full MAX_ITER 10000 of dijet and mlp,
-whose tuning call would otherwise run every shape to the cap.  Synthetic
-data, FP64.
+the line reports the fastest of the timed calls.  With --form the warm-up
+call runs 10 iterations (at most): the way to time the full MAX_ITER 10000 of
+dijet and mlp, whose tuning call would otherwise run every shape to the cap.
+--form thread|tile, or group with --group-threads, fixes the shape; --form
+group alone still lets the tuner choose among the parallel shapes, on that
+short warm-up call.  Synthetic data, FP64.
 """
 from __future__ import annotations
 
