@@ -85,8 +85,9 @@ struct TileLds {
   double fpart[Seg::NP * kTileStarts];
   int cover[S::MB * Seg::kCover]; // parts of each row block, in segment order
   double x[kTileStarts * D], g[kTileStarts * D], p[kTileStarts * D];
-  int live[kTileStarts];
-  int newconv[kTileStarts]; // slot converged in the last round, not yet counted
+  // one byte per slot, so that a wavefront reads all 16 flags in one access
+  alignas(16) uint8_t live[kTileStarts];
+  alignas(16) uint8_t newconv[kTileStarts]; // slot converged in the last round, not yet counted
   int stop; // early stop seen at the start of this round
   TileSlot slot[kTileStarts];
   ProblemDesc prob; // the tile's problem
@@ -487,12 +488,13 @@ template <int D>
 ZEUS_D ZEUS_FI void
 tile_count_converged(TileLds<D>& L, Control* ctl)
 {
-  int n = 0;
-#pragma unroll
-  for (int k = 0; k < kTileStarts; ++k) {
-    n += L.newconv[k];
-    L.newconv[k] = 0;
-  }
+  static_assert(kTileStarts == 16, "the flags are one 16-byte word");
+  uint4* fl = reinterpret_cast<uint4*>(L.newconv);
+  const uint4 v = *fl;
+  *fl = uint4{0u, 0u, 0u, 0u};
+  // bytes are 0 or 1: the byte sum of the four words' sum is the count
+  const unsigned b = v.x + v.y + v.z + v.w;
+  const int n = int((b * 0x01010101u) >> 24);
   if (n > 0)
     (void)__hip_atomic_fetch_add(&ctl->converged, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -595,11 +597,8 @@ bfgs_tile_kernel(TileArgs a)
           L.live[j] = L.slot[j].phase != 0;
       }
       __syncthreads();
-      int any = 0;
-#pragma unroll
-      for (int k = 0; k < kTileStarts; ++k)
-        any |= L.live[k];
-      if (!any)
+      const uint4 lv = *reinterpret_cast<const uint4*>(L.live);
+      if ((lv.x | lv.y | lv.z | lv.w) == 0u)
         break;
     }
     if (tid == kStopReader)
