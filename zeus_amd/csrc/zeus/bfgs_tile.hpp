@@ -124,6 +124,38 @@ tile_dot(const double (&a)[EL], const double (&b)[EL])
   return wave_sum(t);
 }
 
+// Submits slot w's next point x + alpha p into the product's operand Xt.
+// Every row is read before the first store (the compiler cannot tell the LDS
+// rows apart); full rows need no mask (lane < 64).
+template <int D>
+ZEUS_D ZEUS_FI void
+tile_submit(TileLds<D>& L, const int w, const int lane, const double alpha)
+{
+  using S = TileShape<D>;
+  constexpr int EL = S::EL;
+  const double* X = L.x + w * D;
+  const double* Pd = L.p + w * D;
+  double x[EL], p[EL];
+#pragma unroll
+  for (int e = 0; e < EL; ++e)
+    if ((e + 1) * kWave <= D || lane + e * kWave < D) {
+      x[e] = X[lane + e * kWave];
+      p[e] = Pd[lane + e * kWave];
+    }
+#pragma unroll
+  for (int e = 0; e < EL; ++e)
+    if ((e + 1) * kWave <= D || lane + e * kWave < D)
+      L.xt[(lane + e * kWave) * S::XS + w] = ::fma(alpha, p[e], x[e]);
+}
+
+// Armijo acceptance of a line-search trial with value f at step alpha
+// (reference utils.cuh:376-426).
+ZEUS_D ZEUS_FI bool
+tile_accepts(double f, double alpha, double ddir, double best)
+{
+  return f <= ::fma(kArmijoC1 * alpha, ddir, best) && f < best;
+}
+
 // Logic of start slot w for one round, run by one wavefront; `f` is the
 // value at the slot's submitted point.
 template <int D>
@@ -161,20 +193,7 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
       r[e] = ok[e] ? tile_grad<D>(L, row[e], w) : 0.0;
   };
   auto submit = [&](int phase) {
-    // every read before the first store (the compiler cannot tell the LDS
-    // rows apart); full rows need no mask (lane < 64)
-    const double al = st.alpha;
-    double x[EL], p[EL];
-#pragma unroll
-    for (int e = 0; e < EL; ++e)
-      if ((e + 1) * kWave <= D || ok[e]) {
-        x[e] = X[row[e]];
-        p[e] = Pd[row[e]];
-      }
-#pragma unroll
-    for (int e = 0; e < EL; ++e)
-      if ((e + 1) * kWave <= D || ok[e])
-        L.xt[row[e] * S::XS + w] = ::fma(al, p[e], x[e]);
+    tile_submit<D>(L, w, lane, st.alpha);
     st.phase = phase;
   };
   auto finish = [&](int status, double fval, double gnorm) {
@@ -242,7 +261,7 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
   }
   if (st.phase == 2) {
     ++st.ls_evals;
-    const bool accept = f <= ::fma(kArmijoC1 * st.alpha, st.ddir, st.best) && f < st.best;
+    const bool accept = tile_accepts(f, st.alpha, st.ddir, st.best);
     if (!accept) {
       st.alpha *= 0.5;
       if (++st.trial == kLineSearchTrials)
@@ -586,15 +605,31 @@ bfgs_tile_kernel(TileArgs a)
       __syncthreads();
       // ---- start logic: wavefront w runs its slots one after the other ----
       for (int j = w; j < kTileStarts; j += kTileWaves) {
-        if (L.slot[j].phase != 0) {
-          // f = 0.5 sum_p fpart[p]: lane p holds part p's partial, one
-          // fixed-order wavefront sum
-          static_assert(Seg::NP <= kWave, "one part per lane");
-          const double fu = lane < Seg::NP ? L.fpart[lane * kTileStarts + j] : 0.0;
-          tile_slot_step<D>(L, L.slot[j], j, P, a.out, slot_mem(j), 0.5 * wave_sum(fu));
+        TileSlot& sl = L.slot[j];
+        // what a rejected line-search trial needs, read together with the
+        // value's partials: most rounds end that way, and then the slot
+        // takes the backtracking step here (tile_slot_step's, the same
+        // arithmetic) instead of re-reading its state step by step
+        const int ph = sl.phase, tr = sl.trial, le = sl.ls_evals;
+        const double al = sl.alpha, dd = sl.ddir, bs = sl.best;
+        // f = 0.5 sum_p fpart[p]: lane p holds part p's partial, one
+        // fixed-order wavefront sum
+        static_assert(Seg::NP <= kWave, "one part per lane");
+        const double fu = lane < Seg::NP ? L.fpart[lane * kTileStarts + j] : 0.0;
+        if (ph != 0) {
+          const double f = 0.5 * wave_sum(fu);
+          if (ph == 2 && !tile_accepts(f, al, dd, bs)) {
+            sl.ls_evals = le + 1;
+            sl.alpha = al * 0.5;
+            sl.trial = tr + 1;
+            sl.phase = tr + 1 == kLineSearchTrials ? 3 : 2;
+            tile_submit<D>(L, j, lane, al * 0.5);
+          } else {
+            tile_slot_step<D>(L, sl, j, P, a.out, slot_mem(j), f);
+          }
         }
         if (lane == 0)
-          L.live[j] = L.slot[j].phase != 0;
+          L.live[j] = sl.phase != 0;
       }
       __syncthreads();
       const uint4 lv = *reinterpret_cast<const uint4*>(L.live);
