@@ -315,6 +315,17 @@ struct CsFullSeg {
     constexpr int PB = full_detail::part_base<D, NA>(W);
     if constexpr (LEN == 0) // fewer steps than segments: the hook still runs
       hook();
+    // B operands (rows 4 kb + lk of Xt) read kAhead steps before their MFMA,
+    // in a ring of registers, so that the LDS latency overlaps the chain
+    constexpr int kAhead = LEN < 4 ? LEN : 4;
+    auto b_at = [&](int m) {
+      const int kb = m - full_detail::mb_of<D>(m) * S::KB;
+      return xt[(4 * kb + lk) * S::XS + li];
+    };
+    double bq[kAhead > 0 ? kAhead : 1];
+#pragma unroll
+    for (int i = 0; i < kAhead; ++i)
+      bq[i] = b_at(LO + i);
     mfma_acc_t acc = {0.0, 0.0, 0.0, 0.0};
     [&]<int... I>(std::integer_sequence<int, I...>) {
       (
@@ -325,8 +336,12 @@ struct CsFullSeg {
             hook();
             __builtin_amdgcn_sched_barrier(0);
           }
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[I], xt[(4 * kb + lk) * S::XS + li], acc,
-                                                     0, 0, 0);
+          const double b = bq[I % kAhead];
+          if constexpr (I + kAhead < LEN)
+            bq[I % kAhead] = b_at(m + kAhead);
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[I], b, acc, 0, 0, 0);
+          // the read stays kAhead steps early (the scheduler would sink it)
+          __builtin_amdgcn_sched_barrier(0);
           if constexpr (I == LEN - 1 || kb == S::KB - 1) {
             constexpr int part = PB + mb - MB0;
             double* gp = gpart + part * kSlot;
