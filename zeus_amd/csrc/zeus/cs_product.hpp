@@ -317,7 +317,7 @@ struct CsFullSeg {
       hook();
     // B operands (rows 4 kb + lk of Xt) read kAhead steps before their MFMA,
     // in a ring of registers, so that the LDS latency overlaps the chain
-    constexpr int kAhead = LEN < 4 ? LEN : 4;
+    constexpr int kAhead = LEN < 6 ? LEN : 6;
     auto b_at = [&](int m) {
       const int kb = m - full_detail::mb_of<D>(m) * S::KB;
       return xt[(4 * kb + lk) * S::XS + li];
