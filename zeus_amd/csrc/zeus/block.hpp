@@ -26,6 +26,32 @@ readlane_f64(double v, int lane)
   return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
+// v + v[lane ^ kXor] in every lane, kXor = 16 or 32, by the gfx950 lane
+// swaps (v_permlane16_swap: odd rows of the first operand with even rows of
+// the second; v_permlane32_swap: upper half with lower half): register moves,
+// no LDS.  With both operands v, the two results hold, lane by lane, v and
+// its partner's v in some order; IEEE addition is commutative, so every lane
+// gets the bits of v + v[lane ^ kXor].  All lanes must be active.
+template <int kXor>
+ZEUS_D ZEUS_FI double
+xor_sum(double v)
+{
+  static_assert(kXor == 16 || kXor == 32, "row swaps only");
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const unsigned lo = unsigned(b), hi = unsigned(b >> 32);
+  auto swap = [](unsigned x) {
+    if constexpr (kXor == 16)
+      return __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    else
+      return __builtin_amdgcn_permlane32_swap(x, x, false, false);
+  };
+  const auto l = swap(lo);
+  const auto h = swap(hi);
+  const double a = __builtin_bit_cast(double, (uint64_t(h[0]) << 32) | l[0]);
+  const double c = __builtin_bit_cast(double, (uint64_t(h[1]) << 32) | l[1]);
+  return a + c;
+}
+
 // A value every lane of the wavefront holds identically, re-read from the
 // first active lane so that the compiler treats it as wave-uniform (scalar
 // registers) instead of one copy per lane.

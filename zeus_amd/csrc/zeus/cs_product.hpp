@@ -14,6 +14,7 @@
 
 #include <utility>
 
+#include "block.hpp"
 #include "config.hpp"
 
 namespace zeus::dev {
@@ -344,17 +345,26 @@ struct CsFullSeg {
           __builtin_amdgcn_sched_barrier(0);
           if constexpr (I == LEN - 1 || kb == S::KB - 1) {
             constexpr int part = PB + mb - MB0;
+            constexpr bool full = 16 * mb + 16 <= D; // no row of the block past D
             double* gp = gpart + part * kSlot;
+            // the block's x rows read together (clamped into Xt; rows past D
+            // are not used), then the partial value in row order
+            double xr[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int rr = 16 * mb + 4 * q + lk;
+              xr[q] = xt[(full || rr < D ? rr : D - 1) * S::XS + li];
+            }
             double fp = 0.0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const int r = 4 * q + lk, rr = 16 * mb + r;
               gp[r * S::XS + li] = acc[q];
-              if (rr < D)
-                fp = ::fma(xt[rr * S::XS + li], acc[q], fp);
+              if (full || rr < D)
+                fp = ::fma(xr[q], acc[q], fp);
             }
-            fp += __shfl_xor(fp, 16, kWave);
-            fp += __shfl_xor(fp, 32, kWave);
+            fp = xor_sum<16>(fp);
+            fp = xor_sum<32>(fp);
             if (lk == 0)
               fpart[part * kCsTile + li] = fp;
             acc = mfma_acc_t{0.0, 0.0, 0.0, 0.0};
