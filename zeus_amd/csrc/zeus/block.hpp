@@ -26,16 +26,28 @@ readlane_f64(double v, int lane)
   return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
-// v + v[lane ^ kXor] in every lane, kXor = 16 or 32, by the gfx950 lane
-// swaps (v_permlane16_swap: odd rows of the first operand with even rows of
-// the second; v_permlane32_swap: upper half with lower half): register moves,
-// no LDS.  With both operands v, the two results hold, lane by lane, v and
-// its partner's v in some order; IEEE addition is commutative, so every lane
-// gets the bits of v + v[lane ^ kXor].  All lanes must be active.
-template <int kXor>
-ZEUS_D ZEUS_FI double
-xor_sum(double v)
+// One DPP lane exchange of any 64-bit value (two 32-bit moves).
+template <int kCtrl, class T>
+ZEUS_D ZEUS_FI T
+dpp_b64(T v)
 {
+  static_assert(sizeof(T) == 8, "64-bit values");
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(b)), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(b >> 32)), kCtrl, 0xF, 0xF, false);
+  return __builtin_bit_cast(T, (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+}
+
+// The gfx950 row swaps of a 64-bit value with itself (kXor = 16:
+// v_permlane16_swap, odd rows of the first operand with even rows of the
+// second; kXor = 32: v_permlane32_swap, upper half with lower half).  Lane
+// by lane, {first, second} hold v and v[lane ^ kXor] in some order.  All
+// lanes must be active.
+template <int kXor, class T>
+ZEUS_D ZEUS_FI void
+row_swap_b64(T v, T& first, T& second)
+{
+  static_assert(sizeof(T) == 8, "64-bit values");
   static_assert(kXor == 16 || kXor == 32, "row swaps only");
   const uint64_t b = __builtin_bit_cast(uint64_t, v);
   const unsigned lo = unsigned(b), hi = unsigned(b >> 32);
@@ -47,8 +59,22 @@ xor_sum(double v)
   };
   const auto l = swap(lo);
   const auto h = swap(hi);
-  const double a = __builtin_bit_cast(double, (uint64_t(h[0]) << 32) | l[0]);
-  const double c = __builtin_bit_cast(double, (uint64_t(h[1]) << 32) | l[1]);
+  first = __builtin_bit_cast(T, (uint64_t(h[0]) << 32) | l[0]);
+  second = __builtin_bit_cast(T, (uint64_t(h[1]) << 32) | l[1]);
+}
+
+// v + v[lane ^ kXor] in every lane, kXor = 16 or 32, by the gfx950 lane
+// swaps (v_permlane16_swap: odd rows of the first operand with even rows of
+// the second; v_permlane32_swap: upper half with lower half): register moves,
+// no LDS.  With both operands v, the two results hold, lane by lane, v and
+// its partner's v in some order; IEEE addition is commutative, so every lane
+// gets the bits of v + v[lane ^ kXor].  All lanes must be active.
+template <int kXor>
+ZEUS_D ZEUS_FI double
+xor_sum(double v)
+{
+  double a, c;
+  row_swap_b64<kXor>(v, a, c);
   return a + c;
 }
 

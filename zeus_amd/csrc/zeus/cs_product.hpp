@@ -156,15 +156,25 @@ struct CsUpperSeg {
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[I], xt[(4 * kb + lk) * S::XS + li], acc,
                                                      0, 0, 0);
           if constexpr (I == LEN - 1 || m + 1 == upper_detail::off<D>(mb + 1)) {
+            // the block's x rows read together (clamped into Xt; rows past
+            // D are not used), then the partial value in row order, summed
+            // across rows with the lane swaps (as in CsFullSeg)
+            constexpr bool full = 16 * mb + 16 <= D;
+            double xr[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int rr = 16 * mb + 4 * q + lk;
+              xr[q] = xt[(full || rr < D ? rr : D - 1) * S::XS + li];
+            }
             double fp = 0.0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const int rr = 16 * mb + 4 * q + lk;
-              if (rr < D)
-                fp = ::fma(xt[rr * S::XS + li], acc[q], fp);
+              if (full || rr < D)
+                fp = ::fma(xr[q], acc[q], fp);
             }
-            fp += __shfl_xor(fp, 16, kWave);
-            fp += __shfl_xor(fp, 32, kWave);
+            fp = xor_sum<16>(fp);
+            fp = xor_sum<32>(fp);
             if (lk == 0)
               fslot[(W * NRB + (mb - MB0)) * kCsTile + li] = fp;
             acc = mfma_acc_t{0.0, 0.0, 0.0, 0.0};

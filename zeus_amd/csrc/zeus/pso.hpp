@@ -66,18 +66,38 @@ better(double a, int64_t ia, double b, int64_t ib)
   return ia < ib;
 }
 
+// Argmin over the 64 lanes of a wavefront.  `better` is a strict total
+// order on (value, index) pairs of distinct indices, so the winner does not
+// depend on how lanes are paired: DPP exchanges inside each 16-lane row
+// (lane xor 1, xor 2, half-row mirror, row mirror; each step pairs lanes
+// that hold the winners of disjoint lane sets), then the gfx950 row swaps
+// across rows (register moves, no LDS).  Every lane ends with the winner.
+// All lanes must be active.
 ZEUS_D ZEUS_FI void
 wave_argmin(double& v, int64_t& i)
 {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    const double ov = __shfl_xor(v, off, kWave);
-    const int64_t oi = __shfl_xor(i, off, kWave);
+  auto take = [&](double ov, int64_t oi) {
     if (better(ov, oi, v, i)) {
       v = ov;
       i = oi;
     }
-  }
+  };
+  take(dpp_b64<0xB1>(v), dpp_b64<0xB1>(i));   // quad_perm [1,0,3,2]
+  take(dpp_b64<0x4E>(v), dpp_b64<0x4E>(i));   // quad_perm [2,3,0,1]
+  take(dpp_b64<0x141>(v), dpp_b64<0x141>(i)); // row_half_mirror
+  take(dpp_b64<0x140>(v), dpp_b64<0x140>(i)); // row_mirror
+  double v0, v1;
+  int64_t i0, i1;
+  row_swap_b64<16>(v, v0, v1);
+  row_swap_b64<16>(i, i0, i1);
+  v = v0;
+  i = i0;
+  take(v1, i1);
+  row_swap_b64<32>(v, v0, v1);
+  row_swap_b64<32>(i, i0, i1);
+  v = v0;
+  i = i0;
+  take(v1, i1);
 }
 
 // Workgroup argmin; result valid in thread 0.
