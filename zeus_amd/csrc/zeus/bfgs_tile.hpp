@@ -83,7 +83,6 @@ struct TileLds {
   double xt[S::KP * S::XS]; // submitted points, [row][slot]
   double gpart[Seg::NP * Seg::kSlot]; // partial blocks of Cs Xt, [part][row % 16][slot]
   double fpart[Seg::NP * kTileStarts];
-  int cover[S::MB * Seg::kCover]; // parts of each row block, in segment order
   double x[kTileStarts * D], g[kTileStarts * D], p[kTileStarts * D];
   // one byte per slot, so that a wavefront reads all 16 flags in one access
   alignas(16) uint8_t live[kTileStarts];
@@ -94,22 +93,27 @@ struct TileLds {
 };
 
 // Row d of G for slot j: the partial blocks of the segments covering d's row
-// block, added in segment order.
+// block (consecutive parts), added in segment order.  Every part is read at
+// once (past the block's count the first part again, not added).
 template <int D>
 ZEUS_D ZEUS_FI double
 tile_grad(const TileLds<D>& L, int d, int j)
 {
   using Seg = TileSeg<D>;
   using S = TileShape<D>;
-  const int* cv = L.cover + (d >> 4) * Seg::kCover;
-  const int o = (d & 15) * S::XS + j;
-  double g = L.gpart[cv[0] * Seg::kSlot + o];
+  const int mb = (d >> 4) < S::MB - 1 ? (d >> 4) : S::MB - 1;
+  const int p0 = int((Seg::kFirstBits >> (6 * mb)) & 63);
+  const int n = int((Seg::kCountBits >> (4 * mb)) & 15);
+  const double* gp = L.gpart + p0 * Seg::kSlot + (d & 15) * S::XS + j;
+  double v[Seg::kCover];
 #pragma unroll
-  for (int c = 1; c < Seg::kCover; ++c) {
-    const int p = cv[c];
-    if (p >= 0)
-      g += L.gpart[p * Seg::kSlot + o];
-  }
+  for (int c = 0; c < Seg::kCover; ++c)
+    v[c] = gp[(c < n ? c : 0) * Seg::kSlot];
+  double g = v[0];
+#pragma unroll
+  for (int c = 1; c < Seg::kCover; ++c)
+    if (c < n)
+      g += v[c];
   return g;
 }
 
@@ -129,10 +133,13 @@ tile_dot(const double (&a)[EL], const double (&b)[EL])
 // rows apart); full rows need no mask (lane < 64).
 template <int D>
 ZEUS_D ZEUS_FI void
-tile_submit(TileLds<D>& L, const int w, const int lane, const double alpha)
+tile_submit(TileLds<D>& L, const int w, int lane, const double alpha)
 {
   using S = TileShape<D>;
   constexpr int EL = S::EL;
+  // opaque lane: the row addresses are recomputed here, not held (or
+  // spilled) across the round loop
+  asm volatile("" : "+v"(lane));
   const double* X = L.x + w * D;
   const double* Pd = L.p + w * D;
   double x[EL], p[EL];
@@ -187,7 +194,7 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
     for (int e = 0; e < EL; ++e)
       r[e] = ok[e] ? v[row[e]] : 0.0;
   };
-  auto load_grad = [&](double (&r)[EL]) { // G column: k halves in fixed order
+  auto load_grad = [&](double (&r)[EL]) { // G column from its partial blocks
 #pragma unroll
     for (int e = 0; e < EL; ++e)
       r[e] = ok[e] ? tile_grad<D>(L, row[e], w) : 0.0;
@@ -534,8 +541,6 @@ bfgs_tile_kernel(TileArgs a)
   using Seg = TileSeg<D>;
   for (int e = tid; e < S::KP * S::XS; e += kTileThreads)
     L.xt[e] = 0.0; // padding rows stay zero
-  if (tid < S::MB * Seg::kCover)
-    L.cover[tid] = Seg::cover(tid / Seg::kCover, tid % Seg::kCover);
   Seg seg;
 
   for (int64_t tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {

@@ -286,9 +286,9 @@ struct CsFullSeg {
   static constexpr int kCover = full_detail::max_cover<D, NA>();
   double av[LMAX];
 
-  // Part index of cover entry c of row block mb (-1: none).  Plain integer
-  // code, for filling a table once.
-  static ZEUS_HD constexpr int cover(int mb, int c)
+  // Part index of cover entry c of row block mb (-1: none).  Compile-time
+  // only: the kernels use the packed first part and count below.
+  static constexpr int cover(int mb, int c)
   {
     const int w = full_detail::seg_first<D, NA>(mb) + c;
     if (w > full_detail::seg_last<D, NA>(mb))
@@ -296,6 +296,32 @@ struct CsFullSeg {
     return full_detail::part_base<D, NA>(w) + mb -
            full_detail::mb_of<D>(full_detail::lo<D, NA>(w));
   }
+  static constexpr int n_parts(int mb)
+  {
+    return full_detail::seg_last<D, NA>(mb) - full_detail::seg_first<D, NA>(mb) + 1;
+  }
+  // The parts of a row block are consecutive (a segment after the first
+  // that covers the block starts inside it), so a block's parts are its
+  // first part and a count: 6 and 4 bits per row block, packed.
+  static constexpr bool parts_consecutive()
+  {
+    for (int mb = 0; mb < S::MB; ++mb)
+      for (int c = 0; c < n_parts(mb); ++c)
+        if (cover(mb, c) != cover(mb, 0) + c)
+          return false;
+    return true;
+  }
+  static constexpr uint64_t pack(int bits, bool count)
+  {
+    uint64_t v = 0;
+    for (int mb = 0; mb < S::MB; ++mb)
+      v |= uint64_t(count ? n_parts(mb) : cover(mb, 0)) << (bits * mb);
+    return v;
+  }
+  static constexpr uint64_t kFirstBits = pack(6, false);
+  static constexpr uint64_t kCountBits = pack(4, true);
+  static_assert(parts_consecutive(), "a row block's parts are consecutive");
+  static_assert(S::MB * 6 <= 64 && NP <= 64 && kCover < 16, "packed row-block parts");
 
   // Fragments i0 .. i1 - 1 of segment w (wave-uniform); elements past the
   // segment's length are zero and never used.
