@@ -247,9 +247,9 @@ def main() -> int:
                      "flops_per_launch": flops, "ms_per_launch": ms_opt},
         "dual": {"value": value_dual, "ms_per_step": ms_dual,
                  "t_dual_over_t_adjoint": ms_dual / ms_adj,
-                 "converged_per_call": dual[0].c.claimed},
+                 "converged_per_call": float(np.mean([r.c.claimed for r in dual]))},
         "phases_ms": {"pso": float(np.mean([r.ms_pso for r in adj])), "bfgs": ms_opt},
-        "converged_per_call": adj[0].c.claimed,
+        "converged_per_call": conv_adj / len(adj),
         "by_generator": by_gen,
         "no_early_stop": no_stop,
         "graph": graph,
