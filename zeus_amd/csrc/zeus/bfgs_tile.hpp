@@ -189,15 +189,18 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
     row[e] = lane + e * kWave;
     ok[e] = row[e] < D;
   }
+  // rows every lane has (d < 64 (e + 1) <= D) need no mask; the lane index
+  // is opaque to the compiler, so it cannot tell by itself
+  auto okr = [&](int e) { return (e + 1) * kWave <= D || ok[e]; };
   auto load = [&](const double* v, double (&r)[EL]) {
 #pragma unroll
     for (int e = 0; e < EL; ++e)
-      r[e] = ok[e] ? v[row[e]] : 0.0;
+      r[e] = okr(e) ? v[row[e]] : 0.0;
   };
   auto load_grad = [&](double (&r)[EL]) { // G column from its partial blocks
 #pragma unroll
     for (int e = 0; e < EL; ++e)
-      r[e] = ok[e] ? tile_grad<D>(L, row[e], w) : 0.0;
+      r[e] = okr(e) ? tile_grad<D>(L, row[e], w) : 0.0;
   };
   auto submit = [&](int phase) {
     tile_submit<D>(L, w, lane, st.alpha);
@@ -335,8 +338,8 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
     for (int c = 0; c < kC; ++c)
 #pragma unroll
       for (int e = 0; e < EL; ++e) {
-        sj[c][e] = ok[e] ? sj[c][e] : 0.0;
-        aj[c][e] = ok[e] ? aj[c][e] : 0.0;
+        sj[c][e] = okr(e) ? sj[c][e] : 0.0;
+        aj[c][e] = okr(e) ? aj[c][e] : 0.0;
       }
     double cj[kC], dj[kC];
 #pragma unroll
@@ -372,10 +375,10 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
 #pragma unroll
     for (int e = 0; e < EL; ++e) {
-      const double xv = ok[e] ? X[row[e]] : 0.0;
-      const double xtv = ok[e] ? L.xt[row[e] * S::XS + w] : 0.0;
-      const double gv = ok[e] ? G[row[e]] : 0.0;
-      const double pv = ok[e] ? Pd[row[e]] : 0.0;
+      const double xv = okr(e) ? X[row[e]] : 0.0;
+      const double xtv = okr(e) ? L.xt[row[e] * S::XS + w] : 0.0;
+      const double gv = okr(e) ? G[row[e]] : 0.0;
+      const double pv = okr(e) ? Pd[row[e]] : 0.0;
       s[e] = xtv - xv;
       const double y = gn[e] - gv;
       t0 = ::fma(s[e], y, t0);
@@ -399,9 +402,9 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
       double t0 = 0.0, t1 = 0.0;
 #pragma unroll
       for (int e = 0; e < EL; ++e) {
-        const double pv = ok[e] ? Pd[row[e]] : 0.0;
-        a[e] = ok[e] ? ::fma(hb, s[e], -(rho * (v[e] + pv))) : 0.0;
-        const double ga = acc ? gn[e] : (ok[e] ? G[row[e]] : 0.0);
+        const double pv = okr(e) ? Pd[row[e]] : 0.0;
+        a[e] = okr(e) ? ::fma(hb, s[e], -(rho * (v[e] + pv))) : 0.0;
+        const double ga = acc ? gn[e] : (okr(e) ? G[row[e]] : 0.0);
         t0 = ::fma(a[e], ga, t0);
         t1 = ::fma(s[e], ga, t1);
       }
@@ -411,9 +414,9 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
     // p' = -H' g = -(H g + s (a.g) + a (s.g)), H g = v (accepted) or -p
 #pragma unroll
     for (int e = 0; e < EL; ++e) {
-      const double ga = acc ? gn[e] : (ok[e] ? G[row[e]] : 0.0);
+      const double ga = acc ? gn[e] : (okr(e) ? G[row[e]] : 0.0);
       double pn = 0.0;
-      if (ok[e]) {
+      if (okr(e)) {
         const double hg = acc ? v[e] : -Pd[row[e]];
         pn = -::fma(a[e], q1, ::fma(s[e], q0, hg));
         Pd[row[e]] = pn;
@@ -450,7 +453,7 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
     double* sk = hist + int64_t(st.k) * 2 * D;
 #pragma unroll
     for (int e = 0; e < EL; ++e)
-      if (ok[e]) {
+      if (okr(e)) {
         sk[row[e]] = s[e];
         sk[D + row[e]] = a[e];
       }
@@ -458,21 +461,21 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
   } else if (acc) {
 #pragma unroll
     for (int e = 0; e < EL; ++e) {
-      const double pn = ok[e] ? -v[e] : 0.0;
-      if (ok[e])
+      const double pn = okr(e) ? -v[e] : 0.0;
+      if (okr(e))
         Pd[row[e]] = pn;
       dd = ::fma(gn[e], pn, dd);
     }
   } else {
 #pragma unroll
     for (int e = 0; e < EL; ++e)
-      dd = ::fma(ok[e] ? G[row[e]] : 0.0, ok[e] ? Pd[row[e]] : 0.0, dd);
+      dd = ::fma(okr(e) ? G[row[e]] : 0.0, okr(e) ? Pd[row[e]] : 0.0, dd);
   }
   if (acc) {
     st.best = fnew;
 #pragma unroll
     for (int e = 0; e < EL; ++e)
-      if (ok[e]) {
+      if (okr(e)) {
         X[row[e]] = L.xt[row[e] * S::XS + w];
         G[row[e]] = gn[e];
       }
