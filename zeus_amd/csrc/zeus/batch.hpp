@@ -52,8 +52,8 @@ struct ProblemDesc {
   TrajOut traj;           // trajectory recording (traj.fval == nullptr: off)
 };
 
-// Per-problem control block; zeroed before each call.
-struct Control {
+// Per-problem control block; zeroed before each call (reset_control).
+struct ControlHead {
   int32_t stop_flag;       // set once `required` starts have converged
   int32_t converged;       // number of status-1 starts so far
   uint32_t pad0;
@@ -64,6 +64,28 @@ struct Control {
   int32_t best_status, best_iter; // the best start's Result fields
   double best_fval, best_gnorm;
 };
+// The tile form's converged count, split over kConvLines counters 256 bytes
+// apart (workgroup b adds to counter b % kConvLines, one per die under the
+// round-robin dispatch): the tiles of the chip add their counts in the same
+// rounds, and device-scope adds to one address queue behind each other.
+constexpr int kConvLines = 8;
+constexpr int kConvStride = 64; // 32-bit words between counters
+struct Control : ControlHead {
+  int32_t conv_lines[kConvLines * kConvStride];
+};
+
+// Zero a control block (the counters' padding words are never used).
+ZEUS_D ZEUS_FI void
+reset_control(Control* c)
+{
+  ControlHead z{};
+  z.best_idx = -1;
+  z.gbest_val = __builtin_huge_val();
+  *static_cast<ControlHead*>(c) = z;
+#pragma unroll
+  for (int k = 0; k < kConvLines; ++k)
+    c->conv_lines[k * kConvStride] = 0;
+}
 
 // Per-start outputs (structure of arrays over global start index).
 struct StartOut {

@@ -87,7 +87,10 @@ struct TileLds {
   // one byte per slot, so that a wavefront reads all 16 flags in one access
   alignas(16) uint8_t live[kTileStarts];
   alignas(16) uint8_t newconv[kTileStarts]; // slot converged in the last round, not yet counted
-  int stop; // early stop seen at the start of this round
+  // the problem's converged count as read in a round (the reading
+  // wavefronts' counters summed), by round parity: one is summed while the
+  // other, last read in the round before, is cleared
+  int conv_sum[2];
   TileSlot slot[kTileStarts];
   ProblemDesc prob; // the tile's problem
 };
@@ -164,11 +167,12 @@ tile_accepts(double f, double alpha, double ddir, double best)
 }
 
 // Logic of start slot w for one round, run by one wavefront; `f` is the
-// value at the slot's submitted point.
+// value at the slot's submitted point, `stop` the problem's early stop as
+// seen this round.
 template <int D>
 ZEUS_D void
 tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
-               const StartOut& out, double* mem, double f)
+               const StartOut& out, double* mem, double f, const bool stop)
 {
   using S = TileShape<D>;
   constexpr int EL = S::EL;
@@ -242,7 +246,7 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
       finish(kStatusSurrendered, st.best, gnorm_now());
       return;
     }
-    if (L.stop != 0) {
+    if (stop) {
       finish(kStatusStopped, st.best, gnorm_now());
       return;
     }
@@ -499,7 +503,7 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
     finish(kStatusSurrendered, st.best, nrm);
     return;
   }
-  if (L.stop != 0) {
+  if (stop) {
     finish(kStatusStopped, st.best, nrm);
     return;
   }
@@ -525,7 +529,8 @@ tile_count_converged(TileLds<D>& L, Control* ctl)
   const unsigned b = v.x + v.y + v.z + v.w;
   const int n = int((b * 0x01010101u) >> 24);
   if (n > 0)
-    (void)__hip_atomic_fetch_add(&ctl->converged, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    (void)__hip_atomic_fetch_add(&ctl->conv_lines[(blockIdx.x % kConvLines) * kConvStride], n,
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <int D>
@@ -577,9 +582,12 @@ bfgs_tile_kernel(TileArgs a)
     Control* ctl = a.ctl + pi;
     if constexpr (Seg::kKeep > 0) // this problem's fragments kept across rounds
       seg.load(w, Cf, li0, lk0, 0, Seg::kKeep);
+    if (tid < 2)
+      L.conv_sum[tid] = 0;
     __syncthreads();
 
-    for (;;) {
+    int par = 0; // this round's conv_sum (the same in every wavefront)
+    for (;; par ^= 1) {
       // Early stop (reference bfgs_common.cuh:170-208): the problem stops
       // once `required` starts have converged.  Read once per round by the
       // last wavefront (one product unit) before its unit and used after it,
@@ -596,20 +604,26 @@ bfgs_tile_kernel(TileArgs a)
       // third of the way through the product, when the other tiles' sums of
       // the last round have been added, and its latency hides behind the
       // rest of the product.
-      if (tid == kStopReader)
+      if (tid == kStopReader) {
         tile_count_converged(L, ctl);
+        L.conv_sum[par ^ 1] = 0; // last read in the round before
+      }
+      // (wavefront kTileWaves - kConvLines + k reads counter k)
+      constexpr int kReader0 = kTileWaves - kConvLines;
       int conv = 0;
       auto read_count = [&]() {
-        if (tid == kStopReader)
-          conv = __hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (w >= kReader0)
+          conv = __hip_atomic_load(&ctl->conv_lines[(w - kReader0) * kConvStride],
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       };
       // ---- G = Cs Xt in balanced segments, partial blocks and values ----
       if constexpr (Seg::kKeep < Seg::LMAX)
         seg.load(w, Cf, li, lk, Seg::kKeep, Seg::LMAX);
       __builtin_amdgcn_sched_barrier(0);
       seg.mma(w, L.xt, L.gpart, L.fpart, li, lk, read_count);
-      if (tid == kStopReader)
-        L.stop = P.required > 0 && conv >= P.required;
+      if (w >= kReader0 && lane == 0)
+        (void)__hip_atomic_fetch_add(&L.conv_sum[par], conv, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
       __syncthreads();
       // ---- start logic: wavefront w runs its slots one after the other ----
       for (int j = w; j < kTileStarts; j += kTileWaves) {
@@ -633,7 +647,8 @@ bfgs_tile_kernel(TileArgs a)
             sl.phase = tr + 1 == kLineSearchTrials ? 3 : 2;
             tile_submit<D>(L, j, lane, al * 0.5);
           } else {
-            tile_slot_step<D>(L, sl, j, P, a.out, slot_mem(j), f);
+            const bool stop = P.required > 0 && L.conv_sum[par] >= P.required;
+            tile_slot_step<D>(L, sl, j, P, a.out, slot_mem(j), f, stop);
           }
         }
         if (lane == 0)

@@ -15,12 +15,8 @@ static __global__ void
 init_controls_kernel(Control* c, int n, double* gbx, int64_t n_gbx)
 {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) {
-    Control z{};
-    z.best_idx = -1;
-    z.gbest_val = __builtin_huge_val();
-    c[i] = z;
-  }
+  if (i < n)
+    reset_control(c + i);
   for (int64_t k = i; k < n_gbx; k += int64_t(gridDim.x) * blockDim.x)
     gbx[k] = 0.0;
 }
@@ -113,12 +109,8 @@ init_symmetrize_kernel(Control* c, int n, double* gbx, int64_t n_gbx, const Prob
                        int k, double* Cs, double* Cf, int D)
 {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) {
-    Control z{};
-    z.best_idx = -1;
-    z.gbest_val = __builtin_huge_val();
-    c[i] = z;
-  }
+  if (i < n)
+    reset_control(c + i);
   for (int64_t kk = i; kk < n_gbx; kk += int64_t(gridDim.x) * blockDim.x)
     gbx[kk] = 0.0;
   symmetrize_element(desc, k, Cs, Cf, D, int(i));
