@@ -78,6 +78,23 @@ xor_sum(double v)
   return a + c;
 }
 
+// Sum of a 64-bit integer over the 64 lanes, every lane gets it (integer
+// sums are exact in any order): DPP exchanges inside each row, then the row
+// swaps.  All lanes must be active.
+ZEUS_D ZEUS_FI int64_t
+wave_sum_i64(int64_t v)
+{
+  v += dpp_b64<0xB1>(v);  // quad_perm [1,0,3,2]
+  v += dpp_b64<0x4E>(v);  // quad_perm [2,3,0,1]
+  v += dpp_b64<0x141>(v); // row_half_mirror
+  v += dpp_b64<0x140>(v); // row_mirror
+  int64_t a, c;
+  row_swap_b64<16>(v, a, c);
+  v = a + c;
+  row_swap_b64<32>(v, a, c);
+  return a + c;
+}
+
 // A value every lane of the wavefront holds identically, re-read from the
 // first active lane so that the compiler treats it as wave-uniform (scalar
 // registers) instead of one copy per lane.

@@ -198,10 +198,7 @@ summary_kernel(const ProblemDesc* probs, Control* ctl, StartOut out, double* bes
   block_argmin<kSummaryThreads>(v, idx, sv, si); // winner in thread 0
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    int64_t t = cnt[c];
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1)
-      t += __shfl_xor(t, off, kWave);
+    const int64_t t = wave_sum_i64(cnt[c]);
     if ((threadIdx.x & 63) == 0)
       sc[threadIdx.x >> 6][c] = t;
   }
