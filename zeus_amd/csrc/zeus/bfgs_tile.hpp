@@ -513,10 +513,11 @@ tile_slot_step(TileLds<D>& L, TileSlot& st, const int w, const ProblemDesc& P,
   submit(2);
 }
 
-constexpr int kStopReader = kTileThreads - kWave; // thread that counts and reads
+constexpr int kStopReader = kTileThreads - kWave; // thread that adds the tile's count
 
-// Adds the slots flagged in newconv to the problem's converged count (one
-// relaxed atomic, none when no slot converged) and clears the flags.
+// Adds the slots flagged in newconv to the workgroup's counter of the
+// problem's converged count (one relaxed atomic, none when no slot
+// converged) and clears the flags.
 template <int D>
 ZEUS_D ZEUS_FI void
 tile_count_converged(TileLds<D>& L, Control* ctl)
@@ -589,26 +590,24 @@ bfgs_tile_kernel(TileArgs a)
     int par = 0; // this round's conv_sum (the same in every wavefront)
     for (;; par ^= 1) {
       // Early stop (reference bfgs_common.cuh:170-208): the problem stops
-      // once `required` starts have converged.  Read once per round by the
-      // last wavefront (one product unit) before its unit and used after it,
-      // so the load's latency hides behind the MFMA work; a start sees it at
-      // the top of its next iteration, as in the other forms.
+      // once `required` starts have converged; a start sees it at the top of
+      // its next iteration, as in the other forms.
       // per-lane indices made opaque each round: addresses derived from them
       // are recomputed here rather than held across the round loop
       int li = li0, lk = lk0;
       asm volatile("" : "+v"(li), "+v"(lk));
-      // The starts that converged in the last round are added to the
-      // problem's count here, one atomic per tile and round (one per start,
-      // all on one address, queued up behind each other in the rounds where
-      // most starts of the chip converge together).  The count is read a
-      // third of the way through the product, when the other tiles' sums of
-      // the last round have been added, and its latency hides behind the
-      // rest of the product.
+      // The starts that converged in the last round are added to one of the
+      // problem's eight counters here, one atomic per tile and round (one
+      // per start, all on one address, had queued up behind each other in
+      // the rounds where most starts of the chip converge together).
+      // Wavefronts 8-15 each read one counter a third of the way through
+      // their product, when the other tiles' sums of the last round have
+      // been added (the load's latency hides behind the rest of the
+      // product), and add it to this round's LDS sum.
       if (tid == kStopReader) {
         tile_count_converged(L, ctl);
         L.conv_sum[par ^ 1] = 0; // last read in the round before
       }
-      // (wavefront kTileWaves - kConvLines + k reads counter k)
       constexpr int kReader0 = kTileWaves - kConvLines;
       int conv = 0;
       auto read_count = [&]() {
