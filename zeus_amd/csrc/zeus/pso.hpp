@@ -475,7 +475,6 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   __shared__ double xs[S::KP * S::XS]; // positions, [dim][particle]
   __shared__ double fpart[Seg::NF * kPsoTile];
   __shared__ double gb[D];
-  __shared__ double pbv[kPsoTile];
   __shared__ int improved[kPsoTile];
   __shared__ int best_j;
 
@@ -532,36 +531,36 @@ pso_gaussian_mfma_kernel(PsoArgs a)
   seg.mma(w, xs, fpart, li, lk);
   __syncthreads();
 
-  // f_i, pBest on strict improvement
-  if (tid < kPsoTile) {
-    const int64_t sl = base + tid;
-    int imp = 0;
+  // f_i, pBest on strict improvement (lanes 0-15 of wavefront 0), then the
+  // best particle of the workgroup from the same registers (NaN loses, ties
+  // to the lowest index)
+  if (w == 0) {
     double v = __builtin_nan("");
-    if (sl < P.n_starts) {
-      double fv = 0.0; // the upper form carries the factor 0.5
+    if (tid < kPsoTile) {
+      const int64_t sl = base + tid;
+      int imp = 0;
+      if (sl < P.n_starts) {
+        double fv = 0.0; // the upper form carries the factor 0.5
 #pragma unroll
-      for (int k = 0; k < Seg::NF; ++k)
-        fv += fpart[k * kPsoTile + tid];
-      const int64_t s = P.start0 + sl;
-      if (a.iter == 0) {
-        a.pbv[s] = fv;
-        v = fv;
-      } else {
-        v = old_pbv;
-        if (fv < v) {
+        for (int k = 0; k < Seg::NF; ++k)
+          fv += fpart[k * kPsoTile + tid];
+        const int64_t s = P.start0 + sl;
+        if (a.iter == 0) {
           a.pbv[s] = fv;
           v = fv;
-          imp = 1;
+        } else {
+          v = old_pbv;
+          if (fv < v) {
+            a.pbv[s] = fv;
+            v = fv;
+            imp = 1;
+          }
         }
       }
+      improved[tid] = imp;
     }
-    pbv[tid] = v;
-    improved[tid] = imp;
-  }
-  __syncthreads();
-  if (w == 0) { // best particle of the workgroup (NaN loses, ties to the lowest index)
     const bool ok = lane < kPsoTile && base + lane < P.n_starts;
-    double bv = ok ? pbv[lane] : __builtin_nan("");
+    double bv = ok ? v : __builtin_nan("");
     int64_t bi = ok ? base + lane : int64_t(1) << 62;
     wave_argmin(bv, bi);
     if (lane == 0) {
