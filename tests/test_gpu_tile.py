@@ -148,6 +148,30 @@ def test_tile_early_stop(orc, form):
     assert np.all(ps["iter"][st] <= o.iter[st] + 1)
 
 
+def test_tile_early_stop_per_problem():
+    """Three problems of one tile class with their own required counts (one
+    never reached): each problem's converged counters count its own starts
+    only (the tiles of all problems add theirs in the same rounds), its
+    converged count is its status-1 starts, and a problem stops only on its
+    own count."""
+    C = gauss_matrix(150)
+    probs = [z.Problem(z.Gaussian(C), -5, 5, 300, 10000, 10, 10, 1e-8, 42),
+             z.Problem(z.Gaussian(C), -5, 5, 64, 10000, 10, 65, 1e-8, 43, stream=1),
+             z.Problem(z.Gaussian(C), -5, 5, 120, 10000, 10, 2, 1e-8, 44, stream=2)]
+    res = z.run_batch(probs, form="tile", per_start=True)
+    for pr, r in zip(probs, res):
+        st = r.per_start["status"]
+        assert st.shape == (pr.N,)
+        assert r.c.claimed == int((st == 1).sum())
+        assert r.c.stopped == int((st == 2).sum())
+        assert r.c.claimed + r.c.stopped + r.c.surrendered == pr.N
+        if pr.requiredConverged > pr.N:
+            assert r.c.stopped == 0 and r.c.claimed == pr.N
+        else:
+            assert r.c.claimed >= pr.requiredConverged
+        assert r.status == 1 and abs(r.fval) <= F_TOL
+
+
 def test_tile_is_tuned_for_the_headline():
     """The per-workgroup forms of a Gaussian-150 adjoint call include the tile
     form; the autotuner keeps the fastest shape for the size bucket."""
